@@ -1,0 +1,419 @@
+// Device-side tables, argument blocks and helpers shared by the kernels.
+#pragma once
+
+#include "cfd_physics.hpp"
+
+namespace cfd {
+
+constexpr int BLOCK = 256;
+constexpr int SUMM_STRIDE = 16;  // doubles of limiter summary per element
+constexpr int PART_STRIDE = 8;   // doubles of per-block partials
+constexpr int GEO_STRIDE = 16;   // doubles of geometry per element
+constexpr int LIM_STRIDE = 8;    // doubles of limiter geometry per element
+constexpr double INF = __builtin_huge_val();
+
+// Summary layout (per element, written by k_stage, read by k_limit).
+constexpr int SUMM_AVG = 0;    // 4: cell average of the unlimited stage state
+constexpr int SUMM_MID = 4;    // 6: (rho, E) at the three face-midpoint nodes
+constexpr int SUMM_RMIN = 10;  // min nodal density
+constexpr int SUMM_PMIN = 11;  // min nodal pressure
+constexpr int SUMM_FLAGS = 12; // SF_* bits
+constexpr int SF_NEED_RHO = 1;  // some nodal density < epsilon
+constexpr int SF_NEED_P = 2;    // some nodal pressure < epsilon
+constexpr int SF_NONFINITE = 4; // some value is not finite
+
+enum StageMode : int { MODE_RHS = 0, MODE_STAGE1 = 1, MODE_STAGE2 = 2, MODE_STAGE3 = 3, MODE_SUMMARY = 4 };
+
+// Face slot s = 5*f + k -> own node index (face_nodes of the reference
+// element, reference_element.py:274-279; checked on the host at init).
+constexpr int FACE_NODE[NSLOT] = {0, 1, 2, 3, 4, 4, 8, 11, 13, 14, 0, 5, 9, 12, 14};
+constexpr int MID_NODE[3] = {2, 11, 9};  // face_nodes[:, 2] (reference_element.py:295)
+
+constexpr uint64_t pack_face_nodes() {
+  uint64_t v = 0;
+  for (int s = 0; s < NSLOT; ++s) v |= (uint64_t)FACE_NODE[s] << (4 * s);
+  return v;
+}
+constexpr uint64_t FACE_NODE_PACKED = pack_face_nodes();
+
+// For each node, the (up to two) face slots it owns, one byte each, 0xff = none.
+constexpr int node_slots(int j) {
+  int v = 0xffff, t = 0;
+  for (int s = 0; s < NSLOT; ++s)
+    if (FACE_NODE[s] == j) {
+      v = (v & ~(0xff << (8 * t))) | (s << (8 * t));
+      ++t;
+    }
+  return v;
+}
+constexpr int NODE_SLOTS[NP] = {node_slots(0), node_slots(1), node_slots(2),  node_slots(3),  node_slots(4),
+                                node_slots(5), node_slots(6), node_slots(7),  node_slots(8),  node_slots(9),
+                                node_slots(10), node_slots(11), node_slots(12), node_slots(13), node_slots(14)};
+
+__device__ __forceinline__ int face_node_dyn(int s) { return (int)((FACE_NODE_PACKED >> (4 * s)) & 15u); }
+
+extern __constant__ double c_drT[NP * NP];
+extern __constant__ double c_dsT[NP * NP];
+extern __constant__ double c_liftT[NSLOT * NP];
+extern __constant__ double c_avgw[NP];
+extern __constant__ double c_cellw[NP];
+extern __constant__ double c_facew[NSLOT];
+extern __constant__ double c_p1mid[3 * NP];
+extern __constant__ double c_linb[NP * 3];
+
+struct StageArgs {
+  const double* q;
+  const double* q0;
+  double* out;
+  const double* geo;
+  const int* conn;
+  const double* bcs;
+  double* summ;
+  double* bpart;
+  int64_t K;
+  double gamma;
+  double dt;
+};
+
+struct LimitArgs {
+  double* q;
+  const double* summ;
+  const double* geo;
+  const int* conn;
+  const double* lim;
+  const double* bcs;
+  double* part;
+  uint8_t* flags;
+  int64_t K;
+  double gamma;
+  int limiter;
+  int limit_slopes;
+};
+
+struct DtArgs {
+  const double* q;
+  const double* dtg;  // per-element factor (dt kernel) or Jacobian (integral kernel)
+  double* part;
+  int64_t K;
+  double gamma;
+};
+
+struct ElemGeo {
+  double rx, ry, sx, sy;
+  double nx[3], ny[3];
+  double fs[3];  // face_scale = 0.5 |e| / J (dg.py:99)
+  double he[3];  // 0.5 |e|, the edge-weight factor (dg.py:101)
+};
+
+__device__ __forceinline__ void load4(const double* p, double v[4]) {
+  const double2 a = reinterpret_cast<const double2*>(p)[0];
+  const double2 b = reinterpret_cast<const double2*>(p)[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
+__device__ __forceinline__ void store4(double* p, const double v[4]) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+
+__device__ __forceinline__ void load_geo(const double* geo, int64_t e, ElemGeo& g) {
+  double t[16];
+  const double2* p = reinterpret_cast<const double2*>(geo + e * GEO_STRIDE);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double2 v = p[k];
+    t[2 * k] = v.x;
+    t[2 * k + 1] = v.y;
+  }
+  g.rx = t[0];
+  g.ry = t[1];
+  g.sx = t[2];
+  g.sy = t[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    g.nx[f] = t[4 + f];
+    g.ny[f] = t[7 + f];
+    g.fs[f] = t[10 + f];
+    g.he[f] = t[13 + f];
+  }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
+// contiguous range of element blocks so neighbour traces hit its own L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int x = b & 7;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// SSP-RK3 stage combinations with the reference's association
+// (time_integration.py:70,72,74); contraction is off so the bits match.
+template <int MODE>
+__device__ __forceinline__ double rk3_combine(double q0, double qi, double dt, double rhs) {
+#pragma clang fp contract(off)
+  if constexpr (MODE == MODE_STAGE1) {
+    return qi + dt * rhs;
+  } else if constexpr (MODE == MODE_STAGE2) {
+    return 0.75 * q0 + 0.25 * (qi + dt * rhs);
+  } else {
+    return q0 / 3.0 + 2.0 / 3.0 * (qi + dt * rhs);
+  }
+}
+
+__device__ __forceinline__ double reduce_op(int op, double a, double b) {
+  switch (op) {
+    case 0:
+      return a + b;
+    case 1:
+      return nan_min(a, b);
+    case 2:
+      return a < b ? a : b;
+    default:
+      return nan_max(a, b);
+  }
+}
+
+// Block-wide deterministic tree reductions (BLOCK threads).
+__device__ __forceinline__ void block_sum4(const double v[4], double* out) {
+  __shared__ double sh[4][BLOCK];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__device__ __forceinline__ void block_limit_reduce(double rmin, double pmin, double cnt, double adm, double* out) {
+  __shared__ double sh[4][BLOCK];
+  sh[0][threadIdx.x] = rmin;
+  sh[1][threadIdx.x] = pmin;
+  sh[2][threadIdx.x] = cnt;
+  sh[3][threadIdx.x] = adm;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const int t = threadIdx.x;
+      sh[0][t] = nan_min(sh[0][t], sh[0][t + w]);
+      sh[1][t] = nan_min(sh[1][t], sh[1][t + w]);
+      sh[2][t] = sh[2][t] + sh[2][t + w];
+      sh[3][t] = sh[3][t] < sh[3][t + w] ? sh[3][t] : sh[3][t + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__device__ __forceinline__ void block_min1(double v, double* out) {
+  __shared__ double sh[BLOCK];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = nan_min(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// Per-element limiter summary of a stage state held in registers.
+__device__ __forceinline__ void write_summary(const double q[NP][NV], double gamma, double* sm) {
+  const double gm1 = gamma - 1.0;
+  double avg[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < NP; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) avg[c] = fma(c_avgw[i], q[i][c], avg[c]);
+  const double pbar = pressure_of(avg, gm1);
+  const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
+  double rmin = INF, pmin = INF;
+  bool finite = true, need_p = false;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const double p = pressure_of(q[i], gm1);
+    rmin = nan_min(rmin, q[i][0]);
+    pmin = nan_min(pmin, p);
+    need_p = need_p || (p < eps);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
+  }
+  const int flags = ((rmin < eps) ? SF_NEED_RHO : 0) | (need_p ? SF_NEED_P : 0) | (finite ? 0 : SF_NONFINITE);
+  double2* o = reinterpret_cast<double2*>(sm);
+  o[0] = make_double2(avg[0], avg[1]);
+  o[1] = make_double2(avg[2], avg[3]);
+  o[2] = make_double2(q[MID_NODE[0]][0], q[MID_NODE[0]][3]);
+  o[3] = make_double2(q[MID_NODE[1]][0], q[MID_NODE[1]][3]);
+  o[4] = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
+  o[5] = make_double2(rmin, pmin);
+  o[6] = make_double2((double)flags, 0.0);
+  o[7] = make_double2(0.0, 0.0);
+}
+
+// _minmod (limiting.py:124-125).
+__device__ __forceinline__ double minmod2(double a, double b) {
+#pragma clang fp contract(off)
+  if (a * b > 0.0) {
+    const double m = fmin(fabs(a), fabs(b));
+    return a > 0.0 ? m : -m;
+  }
+  return 0.0;
+}
+
+// Troubled-cell detection (limiting.py:128-151). Also returns the scaled
+// neighbour deviations nu * (alpha d_i + beta d_j) for the reconstruction.
+__device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, const double avg[4],
+                                                const double* sm, double sn[3][4]) {
+#pragma clang fp contract(off)
+  const int4 cn = reinterpret_cast<const int4*>(a.conn)[e];
+  const int nbr[3] = {cn.x, cn.y, cn.z};
+  const uint32_t code = (uint32_t)cn.w;
+  const double* lim = a.lim + e * LIM_STRIDE;
+  const double* geo = a.geo + e * GEO_STRIDE;
+  double diff[3][4];
+  int pf[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const uint32_t fc = (code >> (8 * f)) & 0xffu;
+    const int kind = (fc >> 3) & 7;
+    pf[f] = (fc >> 6) & 3;
+    double nb[4];
+    if (kind == BC_INTERIOR) {
+      load4(a.summ + (int64_t)nbr[f] * SUMM_STRIDE, nb);
+    } else if (kind == BC_SLIP) {
+      const double nx = geo[4 + f], ny = geo[7 + f];
+      const double mn = avg[1] * nx + avg[2] * ny;
+      nb[0] = avg[0];
+      nb[1] = avg[1] - 2.0 * mn * nx;
+      nb[2] = avg[2] - 2.0 * mn * ny;
+      nb[3] = avg[3];
+    } else if (kind == BC_FIXED) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) nb[c] = a.bcs[nbr[f] * 4 + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) nb[c] = avg[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) diff[f][c] = nb[c] - avg[c];
+  }
+  bool troubled = false;
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double al = lim[f], be = lim[3 + f];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double paired = diff[0][c];
+      if (pf[f] == 1) paired = diff[1][c];
+      if (pf[f] == 2) paired = diff[2][c];
+      sn[f][c] = 1.5 * (al * diff[f][c] + be * paired);
+    }
+#pragma unroll
+    for (int cc = 0; cc < 2; ++cc) {
+      const int c = cc == 0 ? 0 : 3;
+      const double t = sm[SUMM_MID + 2 * f + cc] - avg[c];
+      const double excess = fabs(minmod2(t, sn[f][c]) - t);
+      troubled = troubled || (excess > 1e-10 * fabs(avg[c]));
+    }
+  }
+  return troubled;
+}
+
+// Cockburn-Shu P1 reconstruction of a troubled cell (limiting.py:153-163).
+__device__ __forceinline__ void reconstruct_p1(double q[NP][4], const double avg[4], const double sn[3][4],
+                                               int limiter) {
+#pragma clang fp contract(off)
+  double d[3][4];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) s = s + c_p1mid[f * NP + i] * q[i][c];
+      const double proj = s - avg[c];
+      if (proj != 0.0) {
+        const double ratio = sn[f][c] / proj;
+        d[f][c] = proj * fmin(1.0, limiter_phi(limiter, ratio));
+      } else {
+        d[f][c] = 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double pos = (fmax(d[0][c], 0.0) + fmax(d[1][c], 0.0)) + fmax(d[2][c], 0.0);
+    const double neg = (fmax(-d[0][c], 0.0) + fmax(-d[1][c], 0.0)) + fmax(-d[2][c], 0.0);
+    const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
+    const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) d[f][c] = tp * fmax(d[f][c], 0.0) - tn * fmax(-d[f][c], 0.0);
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double lin = (c_linb[i * 3 + 0] * d[0][c] + c_linb[i * 3 + 1] * d[1][c]) + c_linb[i * 3 + 2] * d[2][c];
+      q[i][c] = avg[c] + lin;
+    }
+}
+
+// Zhang-Shu positivity scaling (limiting.py:166-200), per element.
+__device__ __forceinline__ void positivity(double q[NP][4], const double avg[4], double gamma) {
+#pragma clang fp contract(off)
+  const double gm1 = gamma - 1.0;
+  const double pbar = pressure_of(avg, gm1);
+  const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
+  double rmin = INF;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) rmin = nan_min(rmin, q[i][0]);
+  if (rmin < eps) {
+    const double span = avg[0] - rmin;
+    double theta = (avg[0] - eps) / span;
+    theta = fmin(fmax(theta, 0.0), 1.0);
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[i][c] = avg[c] + theta * (q[i][c] - avg[c]);
+  }
+  bool any = false;
+  bool need[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    need[i] = pressure_of(q[i], gm1) < eps;
+    any = any || need[i];
+  }
+  if (!any) return;
+  double theta = 1.0;
+  for (int i = 0; i < NP; ++i) {
+    if (!need[i]) continue;
+    double dev[4];
+    for (int c = 0; c < 4; ++c) dev[c] = q[i][c] - avg[c];
+    double lo = 0.0, hi = 1.0;
+    for (int it = 0; it < 60; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      double s[4];
+      for (int c = 0; c < 4; ++c) s[c] = avg[c] + mid * dev[c];
+      if (pressure_of(s, gm1) >= eps) {
+        lo = mid;
+      } else {
+        hi = mid;
+      }
+    }
+    theta = fmin(theta, lo);
+  }
+  if (theta < 1.0) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[i][c] = avg[c] + theta * (q[i][c] - avg[c]);
+  }
+}
+
+}  // namespace cfd

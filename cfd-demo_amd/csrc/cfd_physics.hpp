@@ -1,0 +1,222 @@
+// Pointwise 2-D Euler physics, interface fluxes and limiter functions as
+// gfx950 device functions. Conserved variables are (rho, rho*u, rho*v, E).
+//
+// Formulas follow src/cfd_demo/euler.py:21-61, fluxes.py:24-143 and
+// limiters.py:16-39. The order of operations follows the reference where it
+// decides branches (HLLC wave-speed signs, Roe entropy fix); elsewhere the
+// compiler may contract a*b+c into FMAs, which moves results by at most a
+// few ulps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cfd {
+
+constexpr int NP = 15;       // nodes per degree-4 triangle
+constexpr int NFP = 5;       // nodes per face
+constexpr int NSLOT = 15;    // 3 faces x 5 face nodes
+constexpr int NV = 4;        // conserved variables
+
+enum FluxId : int { FLUX_RUSANOV = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
+enum LimiterId : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_VAN_LEER = 2, LIM_SUPERBEE = 3, LIM_UNLIMITED = 4 };
+enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE = 3 };
+
+constexpr double ROE_ENTROPY_FIX = 0.1;          // fluxes.py:21
+constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.py:24-25
+
+struct Prim {
+  double rho, u, v, p;
+};
+
+// conserved_to_primitive (euler.py:21-27).
+__device__ __forceinline__ Prim to_prim(const double q[4], double gm1) {
+  Prim w;
+  w.rho = q[0];
+  const double inv = 1.0 / q[0];
+  w.u = q[1] * inv;
+  w.v = q[2] * inv;
+  w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
+  return w;
+}
+
+// Exact-division variant used where a branch depends on the value.
+__device__ __forceinline__ Prim to_prim_div(const double q[4], double gm1) {
+  Prim w;
+  w.rho = q[0];
+  w.u = q[1] / q[0];
+  w.v = q[2] / q[0];
+  w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
+  return w;
+}
+
+__device__ __forceinline__ double pressure_of(const double q[4], double gm1) {
+  const double u = q[1] / q[0];
+  const double v = q[2] / q[0];
+  return gm1 * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+}
+
+// normal_flux (euler.py:49-61).
+__device__ __forceinline__ void normal_flux(const double q[4], const Prim& w, double nx, double ny,
+                                            double out[4]) {
+  const double un = w.u * nx + w.v * ny;
+  out[0] = w.rho * un;
+  out[1] = q[1] * un + w.p * nx;
+  out[2] = q[2] * un + w.p * ny;
+  out[3] = (q[3] + w.p) * un;
+}
+
+__device__ __forceinline__ double safe_den(double x) { return x == 0.0 ? TINY : x; }
+
+// Minus-side data computed once per face node and reused by every flux.
+struct Side {
+  double q[4];
+  Prim w;
+  double c;   // sound speed
+  double un;  // normal velocity
+  double f[4];  // F(q) . n
+};
+
+__device__ __forceinline__ void make_side(Side& s, const double q[4], double nx, double ny, double gamma,
+                                          double gm1) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) s.q[c] = q[c];
+  s.w = to_prim(q, gm1);
+  s.c = sqrt(gamma * s.w.p / s.w.rho);
+  s.un = s.w.u * nx + s.w.v * ny;
+  normal_flux(q, s.w, nx, ny, s.f);
+}
+
+// rusanov (fluxes.py:28-37).
+__device__ __forceinline__ void flux_rusanov(const Side& m, const Side& p, double out[4]) {
+  const double sm = fabs(m.un) + m.c;
+  const double sp = fabs(p.un) + p.c;
+  const double alpha = fmax(sm, sp);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = 0.5 * (m.f[c] + p.f[c]) - 0.5 * alpha * (p.q[c] - m.q[c]);
+}
+
+struct RoeAvg {
+  double rho, u, v, h, c;
+};
+
+// _roe_average (fluxes.py:40-51).
+__device__ __forceinline__ RoeAvg roe_average(const Side& m, const Side& p, double gm1) {
+  RoeAvg a;
+  const double hm = (m.q[3] + m.w.p) / m.w.rho;
+  const double hp = (p.q[3] + p.w.p) / p.w.rho;
+  const double wm = sqrt(m.w.rho);
+  const double wp = sqrt(p.w.rho);
+  const double inv = 1.0 / (wm + wp);
+  a.u = (wm * m.w.u + wp * p.w.u) * inv;
+  a.v = (wm * m.w.v + wp * p.w.v) * inv;
+  a.h = (wm * hm + wp * hp) * inv;
+  a.c = sqrt(gm1 * (a.h - 0.5 * (a.u * a.u + a.v * a.v)));
+  a.rho = wm * wp;
+  return a;
+}
+
+// roe (fluxes.py:54-98).
+__device__ __forceinline__ void flux_roe(const Side& m, const Side& p, double nx, double ny, double gm1,
+                                         double out[4]) {
+  const RoeAvg a = roe_average(m, p, gm1);
+  const double un = a.u * nx + a.v * ny;
+  const double ut = -a.u * ny + a.v * nx;
+  const double d_rho = p.w.rho - m.w.rho;
+  const double d_p = p.w.p - m.w.p;
+  const double d_un = p.un - m.un;
+  const double d_ut = (-p.w.u * ny + p.w.v * nx) - (-m.w.u * ny + m.w.v * nx);
+  const double c2 = a.c * a.c;
+  const double inv2c2 = 1.0 / (2.0 * c2);
+  const double a_minus = (d_p - a.rho * a.c * d_un) * inv2c2;
+  const double a_entropy = d_rho - d_p / c2;
+  const double a_shear = a.rho * d_ut;
+  const double a_plus = (d_p + a.rho * a.c * d_un) * inv2c2;
+  const double delta = ROE_ENTROPY_FIX * (fabs(un) + a.c);
+  auto fix = [delta](double lam) {
+    const double mag = fabs(lam);
+    return mag < delta ? (lam * lam + delta * delta) / (2.0 * delta) : mag;
+  };
+  const double lm = fix(un - a.c) * a_minus;
+  const double lp = fix(un + a.c) * a_plus;
+  const double le = fabs(un) * a_entropy;
+  const double ls = fabs(un) * a_shear;
+  const double d0 = lm + le + lp;
+  const double d1 = lm * (a.u - a.c * nx) + le * a.u + ls * (-ny) + lp * (a.u + a.c * nx);
+  const double d2 = lm * (a.v - a.c * ny) + le * a.v + ls * nx + lp * (a.v + a.c * ny);
+  const double d3 = lm * (a.h - a.c * un) + le * (0.5 * (a.u * a.u + a.v * a.v)) + ls * ut + lp * (a.h + a.c * un);
+  out[0] = 0.5 * (m.f[0] + p.f[0]) - 0.5 * d0;
+  out[1] = 0.5 * (m.f[1] + p.f[1]) - 0.5 * d1;
+  out[2] = 0.5 * (m.f[2] + p.f[2]) - 0.5 * d2;
+  out[3] = 0.5 * (m.f[3] + p.f[3]) - 0.5 * d3;
+}
+
+// hllc (fluxes.py:101-143). Only the selected branch's star state is built;
+// the branch order and the >=/> tests are the reference's.
+__device__ __forceinline__ void flux_hllc(const Side& m, const Side& p, double nx, double ny, double gm1,
+                                          double out[4]) {
+  const RoeAvg a = roe_average(m, p, gm1);
+  const double un = a.u * nx + a.v * ny;
+  const double s_m = fmin(m.un - m.c, un - a.c);
+  const double s_p = fmax(p.un + p.c, un + a.c);
+  const double mass_m = m.w.rho * (s_m - m.un);
+  const double mass_p = p.w.rho * (s_p - p.un);
+  const double s_star = (p.w.p - m.w.p + mass_m * m.un - mass_p * p.un) / safe_den(mass_m - mass_p);
+  if (s_m >= 0.0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] = m.f[c];
+    return;
+  }
+  const bool left = s_star >= 0.0;
+  if (!left && !(s_p > 0.0)) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) out[c] = p.f[c];
+    return;
+  }
+  const Side& k = left ? m : p;
+  const double s_k = left ? s_m : s_p;
+  const double mass_k = left ? mass_m : mass_p;
+  const double factor = mass_k / safe_den(s_k - s_star);
+  const double shift = s_star - k.un;
+  double star[4];
+  star[0] = factor;
+  star[1] = factor * (k.w.u + shift * nx);
+  star[2] = factor * (k.w.v + shift * ny);
+  star[3] = factor * (k.q[3] / k.w.rho + shift * (s_star + k.w.p / safe_den(mass_k)));
+#pragma unroll
+  for (int c = 0; c < 4; ++c) out[c] = k.f[c] + s_k * (star[c] - k.q[c]);
+}
+
+template <int FLUX>
+__device__ __forceinline__ void interface_flux(const Side& m, const Side& p, double nx, double ny, double gm1,
+                                               double out[4]) {
+  if constexpr (FLUX == FLUX_RUSANOV) {
+    flux_rusanov(m, p, out);
+  } else if constexpr (FLUX == FLUX_ROE) {
+    flux_roe(m, p, nx, ny, gm1, out);
+  } else {
+    flux_hllc(m, p, nx, ny, gm1, out);
+  }
+}
+
+// Limiter functions phi(r) (limiters.py:16-39), selected at run time.
+__device__ __forceinline__ double limiter_phi(int id, double r) {
+  switch (id) {
+    case LIM_MINMOD:
+      return fmax(0.0, fmin(1.0, r));
+    case LIM_MC:
+      return fmax(0.0, fmin(fmin(2.0 * r, 0.5 * (1.0 + r)), 2.0));
+    case LIM_VAN_LEER:
+      return (r + fabs(r)) / (1.0 + fabs(r));
+    case LIM_SUPERBEE:
+      return fmax(0.0, fmax(fmin(2.0 * r, 1.0), fmin(r, 2.0)));
+    default:
+      return 1.0;
+  }
+}
+
+// NaN-propagating min/max (jnp.min / jnp.max semantics).
+__device__ __forceinline__ double nan_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+}  // namespace cfd

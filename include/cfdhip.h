@@ -1,0 +1,159 @@
+/* C ABI of libcfdhip.so: the gfx950 implementation of tackandr/CFD-demo's
+ * degree-4 nodal DG Euler hot path (RHS, limiting, CFL step, SSP-RK3 step and
+ * the run loop).
+ *
+ * Every entry point returns an int status (CFD_OK on success); the text of
+ * the last error on the calling thread is cfd_last_error(). No C++ exception
+ * crosses this boundary and no function frees memory it did not allocate.
+ *
+ * State arrays use the reference layout: float64, shape (Np=15, K, 4),
+ * C order, conserved variables (rho, rho*u, rho*v, E) on the last axis.
+ * Functions named *_device take HBM pointers (e.g. torch data_ptr()); all
+ * others take host pointers and copy at the call edge.
+ *
+ * A context owns its HBM tables and buffers and one HIP stream. Contexts are
+ * not thread-safe: use one context per host thread.
+ *
+ * The reference binds four jitted callables at src/cfd_demo/solver.py:118-125
+ * (_rhs, _limit, _time_step, _step) and drives them from EulerSolver.run
+ * (solver.py:167-216); the functions below replace exactly those.
+ */
+#ifndef CFDHIP_H
+#define CFDHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CFD_OK = 0,
+  CFD_EINVAL = 1,        /* bad argument -> ValueError */
+  CFD_EHIP = 2,          /* HIP runtime error */
+  CFD_ENOMEM = 3,        /* device allocation failed */
+  CFD_ENOTINIT = 4,      /* cfd_init was not called */
+  CFD_EINITIAL = 10,     /* run: initial state not admissible (solver.py:171-174) */
+  CFD_EINVALID_DT = 11,  /* run: non-finite or non-positive dt (solver.py:182-183) */
+  CFD_EREJECTED = 12     /* run: max_rejections+1 rejections (solver.py:195-197) */
+};
+
+enum { CFD_FLUX_RUSANOV = 0, CFD_FLUX_ROE = 1, CFD_FLUX_HLLC = 2 };
+enum { CFD_LIM_MINMOD = 0, CFD_LIM_MC = 1, CFD_LIM_VAN_LEER = 2, CFD_LIM_SUPERBEE = 3, CFD_LIM_UNLIMITED = 4 };
+
+/* Degree-4 reference-element tables (reference_element.py:264-316 and
+ * limiting.py:97-121); all matrices row-major. */
+typedef struct {
+  const double* dr;             /* 15x15 */
+  const double* ds;             /* 15x15 */
+  const double* lift;           /* 15x15, columns = face slots 5*f+k */
+  const double* cell_weights;   /* 15 */
+  const double* average_weights;/* 15: cell_weights / sum(cell_weights) (limiting.py:107) */
+  const double* face_weights;   /* 15, GLL weight of each face slot */
+  const double* p1_midpoint;    /* 3x15 */
+  const double* linear_basis;   /* 15x3 */
+  const int32_t* face_nodes;    /* 15 */
+  const int32_t* midpoint_nodes;/* 3 */
+} CfdReferenceElement;
+
+/* Compact per-element mesh tables replacing DGOperators (dg.py:30-49) and
+ * LimiterOperators (limiting.py:34-48).
+ *  geo  (K x 16): rx ry sx sy | nx0 nx1 nx2 | ny0 ny1 ny2 | fscale0..2 | 0.5|e|0..2
+ *  conn (K x 4 int32): field f<3 is the neighbour element across face f, or the
+ *       fixed-state index on a FIXED face; field 3 packs one byte per face:
+ *       bits 0-1 neighbour's local face, bit 2 reversed node order,
+ *       bits 3-5 kind (0 interior, 1 slip wall, 2 fixed state, 3 transmissive),
+ *       bits 6-7 Cockburn-Shu partner face j of face i.
+ *  lim  (K x 8): alpha0..2 | beta0..2 | dt_geometry | jacobian
+ *  bc_states (num_bc_states x 4): conserved fixed exterior states. */
+typedef struct {
+  int64_t num_elements;
+  const double* geo;
+  const int32_t* conn;
+  const double* lim;
+  const double* bc_states;
+  int32_t num_bc_states;
+} CfdMeshTables;
+
+/* StepReport (time_integration.py:22-29). */
+typedef struct {
+  double boundary_flux[4];
+  int32_t admissible;
+  double min_density;
+  double min_pressure;
+  int64_t troubled_cells;
+} CfdStepReport;
+
+/* RunResult (solver.py:62-88) plus the values an error message needs. */
+typedef struct {
+  double time;
+  int64_t steps;
+  int64_t rejected_steps;
+  double boundary_flux_integral[4];
+  double min_density;
+  double min_pressure;
+  int64_t max_troubled_cells;
+  double fail_time; /* t at CFD_EINVALID_DT / CFD_EREJECTED */
+  double fail_dt;   /* dt at CFD_EINVALID_DT */
+} CfdRunResult;
+
+typedef struct CfdCtx CfdCtx;
+
+const char* cfd_last_error(void);
+int cfd_device_count(int* count);
+
+/* Upload the reference-element tables to the current device. Replaces the
+ * jnp.asarray copies of build_dg_operators (dg.py:103-114). */
+int cfd_init(const CfdReferenceElement* ref);
+
+/* Replaces EulerSolver.__init__'s table upload and jit binding
+ * (solver.py:94-125). flux/limiter ids follow INTERFACE_FLUXES
+ * (fluxes.py:146-150) and LIMITER_FUNCTIONS (limiters.py:42-48). */
+int cfd_ctx_create(const CfdMeshTables* mesh, int flux_id, int limiter_id, double gamma, double cfl,
+                   int max_rejections, CfdCtx** out);
+int cfd_ctx_destroy(CfdCtx* ctx);
+
+/* _rhs = jit(dg_rhs) (solver.py:118, dg.py:140-165): rhs (15,K,4), bflux (4). */
+int cfd_rhs(CfdCtx* ctx, const double* q, double* rhs, double* boundary_flux);
+int cfd_rhs_device(CfdCtx* ctx, const double* q_dev, double* rhs_dev, double* boundary_flux);
+
+/* _limit = jit(apply_limiters) (solver.py:119-121, limiting.py:203-216).
+ * avg (K,4) and flags (K, 1 = troubled) may be NULL. */
+int cfd_limit(CfdCtx* ctx, const double* q, double* q_out, double* averages, int64_t* troubled_count,
+              uint8_t* flags);
+
+/* _time_step = jit(stable_time_step) (solver.py:122, time_integration.py:37-41). */
+int cfd_time_step(CfdCtx* ctx, const double* q, double* dt);
+
+/* _step = jit(ssp_rk3_step) (solver.py:123-125, time_integration.py:44-84).
+ * flags (3, K) receives each stage's troubled-cell flags; may be NULL. */
+int cfd_step(CfdCtx* ctx, const double* q, double dt, double* q_out, CfdStepReport* report, uint8_t* flags);
+int cfd_step_device(CfdCtx* ctx, const double* q_dev, double dt, double* q_out_dev, CfdStepReport* report);
+
+/* cell_averages (dg.py:168-171) -> (K,4); domain_integral (dg.py:174-176) -> (4). */
+int cfd_cell_averages(CfdCtx* ctx, const double* q, double* averages);
+int cfd_integrate(CfdCtx* ctx, const double* q, double* integral);
+
+/* EulerSolver.run (solver.py:167-216) with the state resident in HBM.
+ * max_steps < 0 means no limit (the reference has none). */
+int cfd_run(CfdCtx* ctx, const double* q0, double final_time, int64_t max_steps, double* q_out,
+            CfdRunResult* result);
+
+/* Resident-state stepping for benchmarks: upload once, then advance. */
+int cfd_state_upload(CfdCtx* ctx, const double* q);
+int cfd_state_download(CfdCtx* ctx, double* q);
+int cfd_state_advance(CfdCtx* ctx, int64_t nsteps, double* dt_used, CfdStepReport* last);
+int cfd_synchronize(CfdCtx* ctx);
+
+/* Pointwise interface flux (fluxes.py:28-143) on n face states: qm, qp (n,4),
+ * nx, ny (n), out (n,4); host arrays. */
+int cfd_flux_pointwise(int flux_id, int64_t n, const double* qm, const double* qp, const double* nx,
+                       const double* ny, double gamma, double* out);
+/* Pointwise limiter function phi(r) (limiters.py:16-39). */
+int cfd_limiter_pointwise(int limiter_id, int64_t n, const double* r, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFDHIP_H */
