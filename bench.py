@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: SSP-RK3 steps of the degree-4 DG Euler solver on one MI355X.
+
+Workload (one "step" = one accepted SSP-RK3 step of the run loop: CFL time
+step, three stages of RHS + combination + limiting, and the step report):
+four-quadrant Riemann problem on [0,1]^2, 1000x1000 rectangles split into
+2,000,000 triangles (120M fp64 DOF), HLLC flux, MC limiter, transmissive
+boundaries, CFL 0.5. The state is resident in HBM before the timed region.
+
+value = DOF-stage updates per second = 3 stages x 60 K DOF x steps / time.
+roofline = the stage kernel (RHS fused with the stage combination), with
+algorithmic bytes (16 + 24 + 24)/3 = 21.33 B per DOF per launch, over its mean
+launch duration from HIP events on the solver's stream.
+cpu_baseline = the NumPy oracle (a restatement of the reference's JAX path;
+the JAX path itself is not available) on a bounded sample of the same
+workload, one core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import os
+
+for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ[_v] = "1"  # the CPU baseline is a one-core NumPy run
+
+import argparse  # noqa: E402
+import ctypes as C  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+from pathlib import Path  # noqa: E402
+
+import numpy as np  # noqa: E402
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cfd-demo_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "p=4 DOF-updates/s (fp64) & % HBM roofline; batched speedup; adjoint/fwd cost"
+HBM_PEAK_GBS = 8000.0
+STAGE_BYTES_PER_DOF = (16.0 + 24.0 + 24.0) / 3.0
+RHS_BYTES_PER_DOF = 16.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    """The oracle's SSP-RK3 step on the same workload at n x n rectangles."""
+    from oracle.mesh import Transmissive, rectangle_mesh
+    from oracle.physics import primitive_to_conserved
+    from oracle.solver import EulerSolver, SolverConfig
+    from cfd_demo_amd.cases import QUADRANT_STATES
+
+    s = EulerSolver(rectangle_mesh(n, n), {t: Transmissive() for t in ("left", "right", "bottom", "top")},
+                    SolverConfig("hllc", "mc", 1.4))
+    cx, cy = s.geometry.centroid.T
+    st = np.empty((s.mesh.num_elements, 4))
+    for key, m in (("NE", (cx > .5) & (cy > .5)), ("NW", (cx < .5) & (cy > .5)), ("SW", (cx < .5) & (cy < .5)),
+                   ("SE", (cx > .5) & (cy < .5))):
+        st[m] = primitive_to_conserved(*QUADRANT_STATES[key], 1.4)
+    q = np.broadcast_to(st[None], s.state_shape).copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        q, rep = s.step(q, s.time_step(q))
+    el = time.perf_counter() - t0
+    dof = 60 * s.mesh.num_elements
+    return {"value": 3 * dof * steps / el, "unit": "DOF-stage-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (NumPy restatement of the reference JAX path) {steps} SSP-RK3 steps incl. CFL dt "
+                      f"on the same four-quadrant HLLC/MC workload at {n}x{n} rectangles ({s.mesh.num_elements} "
+                      f"triangles), {el:.1f} s"}
+
+
+def traffic_from_profile(kernel_prefix: str):
+    p = ROOT / "profiles" / "stage_kernel_c2_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(kernel_prefix)
+    except Exception:
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1000, help="rectangles per side (K = 2 n^2)")
+    ap.add_argument("--cpu-n", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x (1 GiB read + 1 GiB write) through an elementwise kernel")
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported on this build "
+              "(every GPU box has one GPU; see the sharding design for more).", file=sys.stderr)
+        return 2
+
+    from cfd_demo_amd import _lib
+    from cfd_demo_amd.cases import four_quadrant
+
+    t0 = time.perf_counter()
+    case = four_quadrant(args.n, flux="hllc", limiter="mc")
+    solver = case.solver
+    K = solver.mesh.num_elements
+    dof = 60 * K
+    log(f"setup K={K} in {time.perf_counter() - t0:.1f}s")
+    lib = _lib.load()
+    ctx = solver._context()
+    q0 = np.ascontiguousarray(case.initial_state)
+    _lib.check(lib.cfd_state_upload(ctx, _lib.dptr(q0)))
+    rep = _lib.CfdStepReport()
+    dt = C.c_double(0.0)
+    _lib.check(lib.cfd_state_advance(ctx, args.warmup, C.byref(dt), C.byref(rep)))
+    _lib.check(lib.cfd_synchronize(ctx))
+    log(f"warmup {args.warmup} steps done, dt={dt.value:.3e}")
+
+    t0 = time.perf_counter()
+    _lib.check(lib.cfd_state_advance(ctx, args.steps, C.byref(dt), C.byref(rep)))
+    _lib.check(lib.cfd_synchronize(ctx))
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = 3.0 * dof * args.steps / elapsed
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} DOF-stage/s, "
+        f"troubled={rep.troubled_cells}")
+
+    # Kernel durations from HIP events on the solver's stream.
+    _lib.check(lib.cfd_timing_enable(ctx, 1))
+    _lib.check(lib.cfd_state_advance(ctx, 3, C.byref(dt), C.byref(rep)))
+    tim = np.zeros(4)
+    _lib.check(lib.cfd_timing_read(ctx, _lib.dptr(tim)))
+    _lib.check(lib.cfd_timing_enable(ctx, 0))
+    stage_ms = tim[0] / max(tim[1], 1)
+    limit_ms = tim[2] / max(tim[3], 1)
+    rhs_ms = C.c_double(0.0)
+    _lib.check(lib.cfd_time_rhs(ctx, 10, C.byref(rhs_ms)))
+    stage_gbs = STAGE_BYTES_PER_DOF * dof / (stage_ms * 1e-3) / 1e9
+    rhs_gbs = RHS_BYTES_PER_DOF * dof / (rhs_ms.value * 1e-3) / 1e9
+    log(f"stage kernel {stage_ms:.3f} ms ({stage_gbs:.0f} GB/s), limiter {limit_ms:.3f} ms, "
+        f"rhs kernel {rhs_ms.value:.3f} ms ({rhs_gbs:.0f} GB/s)")
+
+    traffic = traffic_from_profile("k_stage")
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-stage-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"four-quadrant Riemann problem, [0,1]^2, {args.n}x{args.n} rectangles split into {K} "
+                        "p=4 triangles (15 nodes), HLLC flux, MC limiter, transmissive sides, SSP-RK3 at CFL 0.5; "
+                        "one step = CFL dt + 3 x (RHS + stage combine + limiting) + report",
+            "elements": K,
+            "dof": dof,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": stage_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": stage_gbs / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_stage<HLLC> (RHS + SSP-RK3 combine + limiter summary), mean of the 3 stage launches",
+            "algorithmic_bytes_per_launch": STAGE_BYTES_PER_DOF * dof,
+            "mean_launch_ms": stage_ms,
+        },
+        "rhs_roofline": {
+            "bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS,
+            "kernel": "k_stage<HLLC, RHS only>", "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
+            "mean_launch_ms": rhs_ms.value,
+        },
+        "limiter_mean_launch_ms": limit_ms,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        log(f"cpu baseline: oracle at {args.cpu_n}x{args.cpu_n}, {args.cpu_steps} steps")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
+        log(f"cpu baseline {line['cpu_baseline']['value']:.4g} DOF-stage/s")
+    print(json.dumps(line), flush=True)
+    solver.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

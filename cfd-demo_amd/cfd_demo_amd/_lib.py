@@ -61,6 +61,7 @@ SIGNATURES = {
     "cfd_ctx_create": (C.c_int, [C.POINTER(CfdMeshTables), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                  C.POINTER(C.c_void_p)]),
     "cfd_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "cfd_ctx_set_stage_block": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "cfd_rhs_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, _dp]),
     "cfd_limit": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _i64p, _u8p]),
@@ -74,6 +75,9 @@ SIGNATURES = {
     "cfd_state_download": (C.c_int, [C.c_void_p, _dp]),
     "cfd_state_advance": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.POINTER(CfdStepReport)]),
     "cfd_synchronize": (C.c_int, [C.c_void_p]),
+    "cfd_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "cfd_timing_read": (C.c_int, [C.c_void_p, _dp]),
+    "cfd_time_rhs": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "cfd_flux_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double, _dp]),
     "cfd_limiter_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp]),
 }

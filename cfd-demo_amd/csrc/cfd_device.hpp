@@ -5,9 +5,12 @@
 
 namespace cfd {
 
-constexpr int BLOCK = 256;
+constexpr int LBLOCK = 256;      // threads per block of the per-element passes
+constexpr int FBLOCK = 64;       // threads per block of the limiter repair pass
+constexpr int FGRID = 1024;      // blocks of the limiter repair pass
 constexpr int SUMM_STRIDE = 16;  // doubles of limiter summary per element
-constexpr int PART_STRIDE = 8;   // doubles of per-block partials
+constexpr int PART_STRIDE = 16;  // doubles per partial row (PART_COLS used)
+constexpr int PART_COLS = 9;
 constexpr int GEO_STRIDE = 16;   // doubles of geometry per element
 constexpr int LIM_STRIDE = 8;    // doubles of limiter geometry per element
 constexpr double INF = __builtin_huge_val();
@@ -18,6 +21,15 @@ constexpr int SUMM_MID = 4;    // 6: (rho, E) at the three face-midpoint nodes
 constexpr int SUMM_RMIN = 10;  // min nodal density
 constexpr int SUMM_PMIN = 11;  // min nodal pressure
 constexpr int SUMM_FLAGS = 12; // SF_* bits
+constexpr int SUMM_LAMBDA = 13;  // max nodal |u| + c (last stage only)
+
+// Partial-row columns: boundary flux (sum), nodal minima, troubled count,
+// admissibility (min of 0/1), CFL candidate dtg / lambda (min).
+constexpr int COL_RMIN = 4;
+constexpr int COL_PMIN = 5;
+constexpr int COL_TROUBLED = 6;
+constexpr int COL_ADM = 7;
+constexpr int COL_DT = 8;
 constexpr int SF_NEED_RHO = 1;  // some nodal density < epsilon
 constexpr int SF_NEED_P = 2;    // some nodal pressure < epsilon
 constexpr int SF_NONFINITE = 4; // some value is not finite
@@ -69,7 +81,7 @@ struct StageArgs {
   const int* conn;
   const double* bcs;
   double* summ;
-  double* bpart;
+  double* part;
   int64_t K;
   double gamma;
   double dt;
@@ -84,10 +96,14 @@ struct LimitArgs {
   const double* bcs;
   double* part;
   uint8_t* flags;
+  int* work;        // worklist of element ids (bit 31 = troubled)
+  int* work_count;  // worklist length, zero on entry to k_limit_detect
+  const double* dtg;
   int64_t K;
   double gamma;
   int limiter;
   int limit_slopes;
+  int want_dt;
 };
 
 struct DtArgs {
@@ -163,67 +179,62 @@ __device__ __forceinline__ double rk3_combine(double q0, double qi, double dt, d
   }
 }
 
-__device__ __forceinline__ double reduce_op(int op, double a, double b) {
-  switch (op) {
-    case 0:
-      return a + b;
-    case 1:
-      return nan_min(a, b);
-    case 2:
-      return a < b ? a : b;
-    default:
-      return nan_max(a, b);
-  }
+// LDS node accessor of the stage kernel (layout [node][half][element]).
+__device__ __forceinline__ void lds_node(const double2* sq, int sb, int j, int t, double v[4]) {
+  const double2 a = sq[(j * 2) * sb + t];
+  const double2 b = sq[(j * 2 + 1) * sb + t];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
 }
 
-// Block-wide deterministic tree reductions (BLOCK threads).
-__device__ __forceinline__ void block_sum4(const double v[4], double* out) {
-  __shared__ double sh[4][BLOCK];
+__device__ __forceinline__ void part_neutral(double row[PART_COLS]) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] = v[c];
+  for (int c = 0; c < PART_COLS; ++c) row[c] = 0.0;
+  row[COL_RMIN] = INF;
+  row[COL_PMIN] = INF;
+  row[COL_ADM] = 1.0;
+  row[COL_DT] = INF;
+}
+
+__device__ __forceinline__ double part_combine(int c, double a, double b) {
+  if (c == COL_RMIN || c == COL_PMIN || c == COL_DT) return nan_min(a, b);
+  if (c == COL_ADM) return a < b ? a : b;
+  return a + b;
+}
+
+// Block-wide deterministic tree reduction of one partial row (NT threads).
+template <int NT>
+__device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], double* out) {
+  __shared__ double sh[PART_COLS][NT];
+#pragma unroll
+  for (int c = 0; c < PART_COLS; ++c) sh[c][threadIdx.x] = row[c];
   __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + w];
+      for (int c = 0; c < PART_COLS; ++c)
+        sh[c][threadIdx.x] = part_combine(c, sh[c][threadIdx.x], sh[c][threadIdx.x + w]);
     }
     __syncthreads();
   }
-  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
+  if ((int)threadIdx.x < PART_COLS) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__device__ __forceinline__ void block_limit_reduce(double rmin, double pmin, double cnt, double adm, double* out) {
-  __shared__ double sh[4][BLOCK];
-  sh[0][threadIdx.x] = rmin;
-  sh[1][threadIdx.x] = pmin;
-  sh[2][threadIdx.x] = cnt;
-  sh[3][threadIdx.x] = adm;
-  __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      const int t = threadIdx.x;
-      sh[0][t] = nan_min(sh[0][t], sh[0][t + w]);
-      sh[1][t] = nan_min(sh[1][t], sh[1][t + w]);
-      sh[2][t] = sh[2][t] + sh[2][t + w];
-      sh[3][t] = sh[3][t] < sh[3][t + w] ? sh[3][t] : sh[3][t + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
-}
-
-__device__ __forceinline__ void block_min1(double v, double* out) {
-  __shared__ double sh[BLOCK];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int w = BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = nan_min(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[0] = sh[0];
+// sqrt(u^2 + v^2) + c with the reference's operation order
+// (time_integration.py:39-40); contraction off so k_dt and the fused stage
+// summary give the same bits as each other.
+__device__ __forceinline__ double wave_speed(const double q[4], double gamma) {
+#pragma clang fp contract(off)
+  const double u = q[1] / q[0];
+  const double v = q[2] / q[0];
+  const double p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  return sqrt(u * u + v * v) + sqrt(gamma * p / q[0]);
 }
 
 // Per-element limiter summary of a stage state held in registers.
+template <bool WITH_LAMBDA>
 __device__ __forceinline__ void write_summary(const double q[NP][NV], double gamma, double* sm) {
   const double gm1 = gamma - 1.0;
   double avg[4] = {0.0, 0.0, 0.0, 0.0};
@@ -233,11 +244,15 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
     for (int c = 0; c < 4; ++c) avg[c] = fma(c_avgw[i], q[i][c], avg[c]);
   const double pbar = pressure_of(avg, gm1);
   const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
-  double rmin = INF, pmin = INF;
+  double rmin = INF, pmin = INF, lam = -INF;
   bool finite = true, need_p = false;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     const double p = pressure_of(q[i], gm1);
+    if constexpr (WITH_LAMBDA) {
+      const double ws = wave_speed(q[i], gamma);
+      lam = i == 0 ? ws : nan_max(lam, ws);
+    }
     rmin = nan_min(rmin, q[i][0]);
     pmin = nan_min(pmin, p);
     need_p = need_p || (p < eps);
@@ -252,7 +267,7 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
   o[3] = make_double2(q[MID_NODE[1]][0], q[MID_NODE[1]][3]);
   o[4] = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
   o[5] = make_double2(rmin, pmin);
-  o[6] = make_double2((double)flags, 0.0);
+  o[6] = make_double2((double)flags, lam);
   o[7] = make_double2(0.0, 0.0);
 }
 

@@ -1,41 +1,64 @@
 // gfx950 kernels for the degree-4 nodal DG Euler hot path:
-//   k_stage   : strong-form RHS (volume + face fluxes + lift), fused with the
-//               SSP-RK3 stage combination and a per-element limiter summary
-//   k_limit   : troubled-cell detection / P1 reconstruction and Zhang-Shu
-//               positivity, in place, touching only elements that need it
-//   k_dt      : CFL time step reduction
-//   k_finalize: deterministic reduction of per-block partials
+//   k_stage        strong-form RHS (volume + face fluxes + lift), fused with
+//                  the SSP-RK3 stage combination and a per-element limiter
+//                  summary (cell average, midpoint traces, minima, flags, and
+//                  in the last stage the CFL wave speed)
+//   k_limit_detect troubled-cell detection from the summaries; elements that
+//                  need repair go to a worklist, the rest report directly
+//   k_limit_fix    P1 reconstruction and Zhang-Shu positivity, in place, for
+//                  worklist elements only
+//   k_dt           CFL time step of a state (first step / time_step API)
+//   k_finalize     deterministic two-level reduction of per-block partials
 //
 // State layout in HBM is the reference's (Np, K, 4) float64, node-major:
-// node j of element e starts at q + (j*K + e)*4. Consecutive lanes own
-// consecutive elements, so every per-node load of a wave is one contiguous
-// 2 KiB stretch. The 15x15 operators live in __constant__ memory and are
-// read with wave-uniform indices, i.e. as scalar operands of the FMAs.
+// node j of element e starts at q + (j*K + e)*4. A stage workgroup owns SB
+// consecutive elements (one lane each), copies their 15 node planes into LDS
+// with coalesced loads, and then reads volume nodes, own face traces,
+// in-block neighbour traces and the stage-combination input from LDS, so each
+// state byte crosses HBM once. The 15x15 operators live in __constant__
+// memory and are read with wave-uniform indices, i.e. as scalar operands.
 #include "cfd_device.hpp"
 
 namespace cfd {
 
-__constant__ double c_drT[NP * NP];    // Dr transposed: column j at [j*NP]
+__constant__ double c_drT[NP * NP];       // Dr transposed: column j at [j*NP]
 __constant__ double c_dsT[NP * NP];
 __constant__ double c_liftT[NSLOT * NP];  // LIFT transposed: column s at [s*NP]
-__constant__ double c_avgw[NP];    // cell weights / sum(cell weights)
-__constant__ double c_cellw[NP];   // cell weights
-__constant__ double c_facew[NSLOT];  // GLL edge weight of each face slot
+__constant__ double c_avgw[NP];           // cell weights / sum(cell weights)
+__constant__ double c_cellw[NP];          // cell weights
+__constant__ double c_facew[NSLOT];       // GLL edge weight of each face slot
 __constant__ double c_p1mid[3 * NP];
 __constant__ double c_linb[NP * 3];
 
 // ---------------------------------------------------------------------------
-// Stage kernel: one lane per element.
+// Stage kernel: one lane per element, SB elements per workgroup.
 // ---------------------------------------------------------------------------
-template <int FLUX, int MODE>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
-k_stage(StageArgs a) {
+template <int FLUX, int MODE, int SB>
+__global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
+  // LDS copy of the block's state: node j, half h (vars 2h, 2h+1), element t.
+  __shared__ double2 sq[NP * 2 * SB];
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t K = a.K;
-  const int64_t e = (int64_t)blk * BLOCK + threadIdx.x;
-  const bool live = e < K;
+  const int64_t e0 = (int64_t)blk * SB;
+  const int ne = (int)((K - e0) < SB ? (K - e0) : SB);
+  const int t = threadIdx.x;
+  const int64_t e = e0 + t;
+  const bool live = t < ne;
   const double gamma = a.gamma;
   const double gm1 = gamma - 1.0;
+
+  // Cooperative, coalesced copy of the block's node planes into LDS.
+  {
+    const double2* src = reinterpret_cast<const double2*>(a.q);
+#pragma unroll 5
+    for (int idx = t; idx < NP * 2 * SB; idx += SB) {
+      const int j = idx / (2 * SB);
+      const int r = idx - j * (2 * SB);
+      const int el = r >> 1, h = r & 1;
+      if (el < ne) sq[(j * 2 + h) * SB + el] = src[((int64_t)j * K + e0 + el) * 2 + h];
+    }
+  }
+  __syncthreads();
 
   double acc[NP][NV];
 #pragma unroll
@@ -47,7 +70,7 @@ k_stage(StageArgs a) {
   if (live) {
     if constexpr (MODE == MODE_SUMMARY) {
 #pragma unroll
-      for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, acc[i]);
+      for (int i = 0; i < NP; ++i) lds_node(sq, SB, i, t, acc[i]);
     } else {
       ElemGeo g;
       load_geo(a.geo, e, g);
@@ -57,11 +80,11 @@ k_stage(StageArgs a) {
 
       // Volume term, node by node; the j loop stays rolled so only one node's
       // fluxes are live next to the 60 accumulators. c_drT/c_dsT/c_liftT are
-      // stored column-major so each column is one run of scalar loads.
+      // column-major so each column is one run of scalar loads.
 #pragma unroll 1
       for (int j = 0; j < NP; ++j) {
         double qj[4];
-        load4(a.q + ((int64_t)j * K + e) * 4, qj);
+        lds_node(sq, SB, j, t, qj);
         const Prim w = to_prim(qj, gm1);
         const double F[4] = {qj[1], qj[1] * w.u + w.p, qj[2] * w.u, (qj[3] + w.p) * w.u};
         const double G[4] = {qj[2], qj[1] * w.v, qj[2] * w.v + w.p, (qj[3] + w.p) * w.v};
@@ -81,8 +104,8 @@ k_stage(StageArgs a) {
         // Face slots whose own node is j (at most two: the corners).
         const int slots = NODE_SLOTS[j];
 #pragma unroll 1
-        for (int t = 0; t < 2; ++t) {
-          const int s = (slots >> (8 * t)) & 0xff;
+        for (int tt = 0; tt < 2; ++tt) {
+          const int s = (slots >> (8 * tt)) & 0xff;
           if (s == 0xff) break;
           const int f = s / NFP;
           const int kk = s - f * NFP;
@@ -106,7 +129,12 @@ k_stage(StageArgs a) {
             const int nf = fc & 3;
             const int pk = (fc & 4) ? (NFP - 1 - kk) : kk;
             const int pn = face_node_dyn(nf * NFP + pk);
-            load4(a.q + ((int64_t)pn * K + nb) * 4, qp);
+            const int64_t loc = (int64_t)nb - e0;
+            if (loc >= 0 && loc < ne) {
+              lds_node(sq, SB, pn, (int)loc, qp);
+            } else {
+              load4(a.q + ((int64_t)pn * K + nb) * 4, qp);
+            }
           } else if (kind == BC_SLIP) {
             const double mn = qj[1] * nx + qj[2] * ny;
             qp[0] = qj[0];
@@ -150,7 +178,7 @@ k_stage(StageArgs a) {
           store4(o, acc[i]);
         } else {
           double qi[4];
-          load4(a.q + ((int64_t)i * K + e) * 4, qi);
+          lds_node(sq, SB, i, t, qi);
           double q0[4] = {0.0, 0.0, 0.0, 0.0};
           if constexpr (MODE != MODE_STAGE1) load4(a.q0 + ((int64_t)i * K + e) * 4, q0);
 #pragma unroll
@@ -161,99 +189,128 @@ k_stage(StageArgs a) {
     }
   }
 
-  if constexpr (MODE == MODE_RHS) {
-    block_sum4(bfl, a.bpart + (int64_t)blk * PART_STRIDE);
-  } else {
-    if (live) write_summary(acc, gamma, a.summ + e * SUMM_STRIDE);
-    if constexpr (MODE != MODE_SUMMARY) block_sum4(bfl, a.bpart + (int64_t)blk * PART_STRIDE);
+  if constexpr (MODE != MODE_RHS) {
+    if (live) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ + e * SUMM_STRIDE);
+  }
+  if constexpr (MODE != MODE_SUMMARY) {
+    double row[PART_COLS];
+    part_neutral(row);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) row[c] = bfl[c];
+    block_reduce_row<SB>(row, a.part + (int64_t)blk * PART_STRIDE);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Limiter kernel: one lane per element, in place on the stage state.
+// Limiter, pass 1: detection for every element from the stage summaries.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK) k_limit(LimitArgs a) {
+__global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t K = a.K;
-  const int64_t e = (int64_t)blk * BLOCK + threadIdx.x;
-  double rmin = INF, pmin = INF;
-  double troubled_count = 0.0, adm = 1.0;
-  if (e < K) {
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
+  if (e < a.K) {
     const double* sm = a.summ + e * SUMM_STRIDE;
     double avg[4];
     load4(sm, avg);
-    const double flags = sm[SUMM_FLAGS];
-    const int fl = (int)flags;
-    const double gm1 = a.gamma - 1.0;
+    const int fl = (int)sm[SUMM_FLAGS];
     bool troubled = false;
-    double sn[3][4];
-    if (a.limit_slopes) troubled = detect_troubled(a, e, avg, sm, sn);
-    const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
-    if (work) {
-      double q[NP][4];
-      double* base = a.q;
-#pragma unroll
-      for (int i = 0; i < NP; ++i) load4(base + ((int64_t)i * K + e) * 4, q[i]);
-      if (troubled) reconstruct_p1(q, avg, sn, a.limiter);
-      positivity(q, avg, a.gamma);
-      bool finite = true;
-      rmin = INF;
-      pmin = INF;
-#pragma unroll
-      for (int i = 0; i < NP; ++i) {
-        store4(base + ((int64_t)i * K + e) * 4, q[i]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
-        rmin = nan_min(rmin, q[i][0]);
-        pmin = nan_min(pmin, pressure_of(q[i], gm1));
-      }
-      adm = finite ? 1.0 : 0.0;
-    } else {
-      rmin = sm[SUMM_RMIN];
-      pmin = sm[SUMM_PMIN];
-      adm = (fl & SF_NONFINITE) ? 0.0 : 1.0;
+    if (a.limit_slopes) {
+      double sn[3][4];
+      troubled = detect_troubled(a, e, avg, sm, sn);
     }
-    const double pbar = pressure_of(avg, gm1);
-    if (!(avg[0] > 0.0) || !(pbar > 0.0)) adm = 0.0;
-    troubled_count = troubled ? 1.0 : 0.0;
+    const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
+    const double pbar = pressure_of(avg, a.gamma - 1.0);
+    const bool avg_ok = (avg[0] > 0.0) && (pbar > 0.0);
+    row[COL_TROUBLED] = troubled ? 1.0 : 0.0;
+    if (work) {
+      const int slot = atomicAdd(a.work_count, 1);
+      a.work[slot] = (int)e | (troubled ? (int)0x80000000u : 0);
+      row[COL_ADM] = avg_ok ? 1.0 : 0.0;  // finiteness is checked by k_limit_fix
+    } else {
+      row[COL_RMIN] = sm[SUMM_RMIN];
+      row[COL_PMIN] = sm[SUMM_PMIN];
+      row[COL_ADM] = (avg_ok && !(fl & SF_NONFINITE)) ? 1.0 : 0.0;
+      if (a.want_dt) row[COL_DT] = a.dtg[e] / sm[SUMM_LAMBDA];
+    }
     if (a.flags) a.flags[e] = troubled ? 1 : 0;
   }
-  block_limit_reduce(rmin, pmin, troubled_count, adm, a.part + (int64_t)blk * PART_STRIDE + 4);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
+}
+
+// ---------------------------------------------------------------------------
+// Limiter, pass 2: reconstruction and positivity for worklist elements. A
+// fixed grid strides over the worklist; every lane reaches the exit.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
+  double row[PART_COLS];
+  part_neutral(row);
+  const int n = *a.work_count;
+  const double gm1 = a.gamma - 1.0;
+  for (int idx = blockIdx.x * FBLOCK + threadIdx.x; idx < n; idx += gridDim.x * FBLOCK) {
+    const int entry = a.work[idx];
+    const int64_t e = entry & 0x7fffffff;
+    const bool troubled = entry < 0;
+    const int64_t K = a.K;
+    const double* sm = a.summ + e * SUMM_STRIDE;
+    double avg[4];
+    load4(sm, avg);
+    double q[NP][4];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, q[i]);
+    if (troubled) {
+      double sn[3][4];
+      detect_troubled(a, e, avg, sm, sn);
+      reconstruct_p1(q, avg, sn, a.limiter);
+    }
+    positivity(q, avg, a.gamma);
+    bool finite = true;
+    double rmin = INF, pmin = INF, lam = -INF;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      store4(a.q + ((int64_t)i * K + e) * 4, q[i]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
+      rmin = nan_min(rmin, q[i][0]);
+      pmin = nan_min(pmin, pressure_of(q[i], gm1));
+      if (a.want_dt) lam = i == 0 ? wave_speed(q[i], a.gamma) : nan_max(lam, wave_speed(q[i], a.gamma));
+    }
+    row[COL_RMIN] = nan_min(row[COL_RMIN], rmin);
+    row[COL_PMIN] = nan_min(row[COL_PMIN], pmin);
+    if (!finite) row[COL_ADM] = 0.0;
+    if (a.want_dt) row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / lam);
+  }
+  block_reduce_row<FBLOCK>(row, a.part + (int64_t)blockIdx.x * PART_STRIDE);
 }
 
 // ---------------------------------------------------------------------------
 // CFL time step: min over elements of dtg / max_i(|u| + c) (time_integration.py:37-41).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK) k_dt(DtArgs a) {
+__global__ void __launch_bounds__(LBLOCK) k_dt(DtArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t e = (int64_t)blk * BLOCK + threadIdx.x;
-  double val = INF;
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
   if (e < a.K) {
-#pragma clang fp contract(off)
-    const double gm1 = a.gamma - 1.0;
     double lam = -INF;
-    bool first = true;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       double q[4];
       load4(a.q + ((int64_t)i * a.K + e) * 4, q);
-      const double u = q[1] / q[0];
-      const double v = q[2] / q[0];
-      const double p = gm1 * (q[3] - 0.5 * (q[1] * u + q[2] * v));
-      const double ws = sqrt(u * u + v * v) + sqrt(a.gamma * p / q[0]);
-      lam = first ? ws : nan_max(lam, ws);
-      first = false;
+      const double ws = wave_speed(q, a.gamma);
+      lam = i == 0 ? ws : nan_max(lam, ws);
     }
-    val = a.dtg[e] / lam;
+    row[COL_DT] = a.dtg[e] / lam;
   }
-  block_min1(val, a.part + (int64_t)blk * PART_STRIDE);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
-// Initial-state check (solver.py:171-176): per block [rho_min, p_min, all_ok].
-__global__ void __launch_bounds__(BLOCK) k_check(DtArgs a) {
+// Initial-state check (solver.py:171-176): rho_min, p_min and "all admissible".
+__global__ void __launch_bounds__(LBLOCK) k_check(DtArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t e = (int64_t)blk * BLOCK + threadIdx.x;
-  double rmin = INF, pmin = INF, ok = 1.0;
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
   if (e < a.K) {
     const double gm1 = a.gamma - 1.0;
 #pragma unroll
@@ -264,20 +321,22 @@ __global__ void __launch_bounds__(BLOCK) k_check(DtArgs a) {
       bool fin = true;
 #pragma unroll
       for (int c = 0; c < 4; ++c) fin = fin && isfinite(q[c]);
-      if (!(fin && q[0] > 0.0 && p > 0.0)) ok = 0.0;
-      rmin = nan_min(rmin, q[0]);
-      pmin = nan_min(pmin, p);
+      if (!(fin && q[0] > 0.0 && p > 0.0)) row[COL_ADM] = 0.0;
+      row[COL_RMIN] = nan_min(row[COL_RMIN], q[0]);
+      row[COL_PMIN] = nan_min(row[COL_PMIN], p);
     }
   }
-  block_limit_reduce(rmin, pmin, 0.0, ok, a.part + (int64_t)blk * PART_STRIDE);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
 // Domain integral sum_k J_k sum_i w_i q[i,k,:] (dg.py:174-176), per block.
-__global__ void __launch_bounds__(BLOCK) k_integrate(DtArgs a) {
+__global__ void __launch_bounds__(LBLOCK) k_integrate(DtArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t e = (int64_t)blk * BLOCK + threadIdx.x;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
   if (e < a.K) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       double q[4];
@@ -285,31 +344,47 @@ __global__ void __launch_bounds__(BLOCK) k_integrate(DtArgs a) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) s[c] += c_cellw[i] * q[c];
     }
-    const double J = a.dtg[e];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s[c] *= J;
+    for (int c = 0; c < 4; ++c) row[c] = s[c] * a.dtg[e];
   }
-  block_sum4(s, a.part + (int64_t)blk * PART_STRIDE);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
-// Deterministic reduction of nb partial rows (PART_STRIDE doubles each).
-// ops[c]: 0 = sum, 1 = nan-min, 2 = and (min of 0/1), 3 = max.
-__global__ void __launch_bounds__(BLOCK) k_finalize(const double* __restrict__ part, int nb, int ncols,
-                                                   int4 ops_lo, int4 ops_hi, double* __restrict__ out) {
-  __shared__ double sh[BLOCK];
-  const int ops[8] = {ops_lo.x, ops_lo.y, ops_lo.z, ops_lo.w, ops_hi.x, ops_hi.y, ops_hi.z, ops_hi.w};
-  for (int c = 0; c < ncols; ++c) {
-    const int op = ops[c];
-    double v = (op == 0) ? 0.0 : (op == 1 || op == 2) ? INF : -INF;
-    for (int b = threadIdx.x; b < nb; b += BLOCK) v = reduce_op(op, v, part[(int64_t)b * PART_STRIDE + c]);
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-      if (threadIdx.x < w) sh[threadIdx.x] = reduce_op(op, sh[threadIdx.x], sh[threadIdx.x + w]);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = sh[0];
-    __syncthreads();
+// Deterministic reduction of nrows partial rows into out[PART_COLS]. Each
+// block reduces a fixed contiguous chunk; the last block to finish reduces
+// the chunk results in order, writes out, and re-arms the counters (the
+// block-completion counter and, if given, the limiter worklist counter).
+__global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ part, int nrows,
+                                                    double* __restrict__ chunk, int* done, int* work_count,
+                                                    double* __restrict__ out) {
+  __shared__ int last;
+  const int per = (nrows + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per;
+  const int r1 = min(nrows, r0 + per);
+  double row[PART_COLS];
+  part_neutral(row);
+  for (int r = r0 + threadIdx.x; r < r1; r += LBLOCK) {
+    const double* p = part + (int64_t)r * PART_STRIDE;
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+  }
+  block_reduce_row<LBLOCK>(row, chunk + (int64_t)blockIdx.x * PART_STRIDE);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(done, 1) == (int)gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  part_neutral(row);
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += LBLOCK) {
+    const volatile double* p = chunk + (int64_t)b * PART_STRIDE;
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+  }
+  block_reduce_row<LBLOCK>(row, out);
+  if (threadIdx.x == 0) {
+    *done = 0;
+    if (work_count) *work_count = 0;
   }
 }
 

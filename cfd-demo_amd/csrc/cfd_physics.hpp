@@ -50,7 +50,9 @@ __device__ __forceinline__ Prim to_prim_div(const double q[4], double gm1) {
   return w;
 }
 
+// pressure (euler.py:30-32) with the reference's rounding: contraction off.
 __device__ __forceinline__ double pressure_of(const double q[4], double gm1) {
+#pragma clang fp contract(off)
   const double u = q[1] / q[0];
   const double v = q[2] / q[0];
   return gm1 * (q[3] - 0.5 * (q[1] * u + q[2] * v));

@@ -112,6 +112,9 @@ int cfd_init(const CfdReferenceElement* ref);
 int cfd_ctx_create(const CfdMeshTables* mesh, int flux_id, int limiter_id, double gamma, double cfl,
                    int max_rejections, CfdCtx** out);
 int cfd_ctx_destroy(CfdCtx* ctx);
+/* Tuning: elements per stage-kernel workgroup (64, 128 or 256; default 64,
+ * or the CFDHIP_STAGE_BLOCK environment variable). No reference counterpart. */
+int cfd_ctx_set_stage_block(CfdCtx* ctx, int elements_per_workgroup);
 
 /* _rhs = jit(dg_rhs) (solver.py:118, dg.py:140-165): rhs (15,K,4), bflux (4). */
 int cfd_rhs(CfdCtx* ctx, const double* q, double* rhs, double* boundary_flux);
@@ -144,6 +147,15 @@ int cfd_state_upload(CfdCtx* ctx, const double* q);
 int cfd_state_download(CfdCtx* ctx, double* q);
 int cfd_state_advance(CfdCtx* ctx, int64_t nsteps, double* dt_used, CfdStepReport* last);
 int cfd_synchronize(CfdCtx* ctx);
+
+/* Kernel timing with HIP events on the context's stream (benchmarks):
+ * cfd_timing_read gives [stage-kernel ms total, launches, limiter ms total,
+ * launches] accumulated by cfd_state_advance since cfd_timing_enable;
+ * cfd_time_rhs gives the mean duration of the RHS-only kernel on the
+ * resident state over reps launches. */
+int cfd_timing_enable(CfdCtx* ctx, int on);
+int cfd_timing_read(CfdCtx* ctx, double* out4);
+int cfd_time_rhs(CfdCtx* ctx, int reps, double* avg_ms);
 
 /* Pointwise interface flux (fluxes.py:28-143) on n face states: qm, qp (n,4),
  * nx, ny (n), out (n,4); host arrays. */
