@@ -179,16 +179,6 @@ __device__ __forceinline__ double rk3_combine(double q0, double qi, double dt, d
   }
 }
 
-// LDS node accessor of the stage kernel (layout [node][half][element]).
-__device__ __forceinline__ void lds_node(const double2* sq, int sb, int j, int t, double v[4]) {
-  const double2 a = sq[(j * 2) * sb + t];
-  const double2 b = sq[(j * 2 + 1) * sb + t];
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
-}
-
 __device__ __forceinline__ void part_neutral(double row[PART_COLS]) {
 #pragma unroll
   for (int c = 0; c < PART_COLS; ++c) row[c] = 0.0;
@@ -204,22 +194,32 @@ __device__ __forceinline__ double part_combine(int c, double a, double b) {
   return a + b;
 }
 
-// Block-wide deterministic tree reduction of one partial row (NT threads).
+// Block-wide deterministic reduction of one partial row (NT threads): a fixed
+// butterfly inside each wave, then wave results in order through LDS.
 template <int NT>
 __device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], double* out) {
-  __shared__ double sh[PART_COLS][NT];
+  constexpr int NW = NT / 64;
+  __shared__ double sh[NW][PART_COLS];
+  double v[PART_COLS];
 #pragma unroll
-  for (int c = 0; c < PART_COLS; ++c) sh[c][threadIdx.x] = row[c];
+  for (int c = 0; c < PART_COLS; ++c) v[c] = row[c];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) v[c] = part_combine(c, v[c], __shfl_xor(v[c], off, 64));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) sh[w][c] = v[c];
   __syncthreads();
-  for (int w = NT / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
+  if (threadIdx.x == 0) {
 #pragma unroll
-      for (int c = 0; c < PART_COLS; ++c)
-        sh[c][threadIdx.x] = part_combine(c, sh[c][threadIdx.x], sh[c][threadIdx.x + w]);
+    for (int c = 0; c < PART_COLS; ++c) {
+      double t = sh[0][c];
+      for (int k = 1; k < NW; ++k) t = part_combine(c, t, sh[k][c]);
+      out[c] = t;
     }
-    __syncthreads();
   }
-  if ((int)threadIdx.x < PART_COLS) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 // sqrt(u^2 + v^2) + c with the reference's operation order
@@ -231,6 +231,16 @@ __device__ __forceinline__ double wave_speed(const double q[4], double gamma) {
   const double v = q[2] / q[0];
   const double p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
   return sqrt(u * u + v * v) + sqrt(gamma * p / q[0]);
+}
+
+// pressure (euler.py:30-32) and sqrt(u^2 + v^2) + c from one pair of
+// divisions; bit-identical to pressure_of and wave_speed.
+__device__ __forceinline__ void pressure_and_speed(const double q[4], double gamma, double& p, double& ws) {
+#pragma clang fp contract(off)
+  const double u = q[1] / q[0];
+  const double v = q[2] / q[0];
+  p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * u + q[2] * v));
+  ws = sqrt(u * u + v * v) + sqrt(gamma * p / q[0]);
 }
 
 // Per-element limiter summary of a stage state held in registers.
@@ -248,10 +258,13 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
   bool finite = true, need_p = false;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
-    const double p = pressure_of(q[i], gm1);
+    double p;
     if constexpr (WITH_LAMBDA) {
-      const double ws = wave_speed(q[i], gamma);
+      double ws;
+      pressure_and_speed(q[i], gamma, p, ws);
       lam = i == 0 ? ws : nan_max(lam, ws);
+    } else {
+      p = pressure_of(q[i], gm1);
     }
     rmin = nan_min(rmin, q[i][0]);
     pmin = nan_min(pmin, p);
@@ -291,7 +304,9 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
   const uint32_t code = (uint32_t)cn.w;
   const double* lim = a.lim + e * LIM_STRIDE;
   const double* geo = a.geo + e * GEO_STRIDE;
-  double diff[3][4];
+  // Neighbour-average differences per face, kept as three separate arrays so
+  // the partner-face selection below stays a register select.
+  double d0[4], d1[4], d2[4];
   int pf[3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
@@ -315,19 +330,24 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
 #pragma unroll
       for (int c = 0; c < 4; ++c) nb[c] = avg[c];
     }
+    double* d = f == 0 ? d0 : (f == 1 ? d1 : d2);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) diff[f][c] = nb[c] - avg[c];
+    for (int c = 0; c < 4; ++c) d[c] = nb[c] - avg[c];
   }
   bool troubled = false;
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     const double al = lim[f], be = lim[3 + f];
+    const double* dself = f == 0 ? d0 : (f == 1 ? d1 : d2);
+    // The partner face is one of the two other faces.
+    const int fa = (f + 1) % 3, fb = (f + 2) % 3;
+    const double* da = fa == 0 ? d0 : (fa == 1 ? d1 : d2);
+    const double* db = fb == 0 ? d0 : (fb == 1 ? d1 : d2);
+    const bool use_a = pf[f] == fa;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      double paired = diff[0][c];
-      if (pf[f] == 1) paired = diff[1][c];
-      if (pf[f] == 2) paired = diff[2][c];
-      sn[f][c] = 1.5 * (al * diff[f][c] + be * paired);
+      const double paired = use_a ? da[c] : db[c];
+      sn[f][c] = 1.5 * (al * dself[c] + be * paired);
     }
 #pragma unroll
     for (int cc = 0; cc < 2; ++cc) {

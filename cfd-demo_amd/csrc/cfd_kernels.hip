@@ -12,11 +12,12 @@
 //
 // State layout in HBM is the reference's (Np, K, 4) float64, node-major:
 // node j of element e starts at q + (j*K + e)*4. A stage workgroup owns SB
-// consecutive elements (one lane each), copies their 15 node planes into LDS
-// with coalesced loads, and then reads volume nodes, own face traces,
-// in-block neighbour traces and the stage-combination input from LDS, so each
-// state byte crosses HBM once. The 15x15 operators live in __constant__
-// memory and are read with wave-uniform indices, i.e. as scalar operands.
+// consecutive elements, one lane each, so every per-node load of a wave is
+// one contiguous 2 KiB run. The 15x15 operators live in __constant__ memory
+// and are read with wave-uniform indices, i.e. as scalar FMA operands; the 60
+// accumulators stay in registers (two waves per SIMD). Staging the block's
+// state in LDS (one wave per SIMD) and a 16-lanes-per-element FP64 MFMA tile
+// variant were both measured slower on MI355X and are not kept.
 #include "cfd_device.hpp"
 
 namespace cfd {
@@ -34,9 +35,7 @@ __constant__ double c_linb[NP * 3];
 // Stage kernel: one lane per element, SB elements per workgroup.
 // ---------------------------------------------------------------------------
 template <int FLUX, int MODE, int SB>
-__global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
-  // LDS copy of the block's state: node j, half h (vars 2h, 2h+1), element t.
-  __shared__ double2 sq[NP * 2 * SB];
+__global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_stage(StageArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t K = a.K;
   const int64_t e0 = (int64_t)blk * SB;
@@ -46,19 +45,6 @@ __global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
   const bool live = t < ne;
   const double gamma = a.gamma;
   const double gm1 = gamma - 1.0;
-
-  // Cooperative, coalesced copy of the block's node planes into LDS.
-  {
-    const double2* src = reinterpret_cast<const double2*>(a.q);
-#pragma unroll 5
-    for (int idx = t; idx < NP * 2 * SB; idx += SB) {
-      const int j = idx / (2 * SB);
-      const int r = idx - j * (2 * SB);
-      const int el = r >> 1, h = r & 1;
-      if (el < ne) sq[(j * 2 + h) * SB + el] = src[((int64_t)j * K + e0 + el) * 2 + h];
-    }
-  }
-  __syncthreads();
 
   double acc[NP][NV];
 #pragma unroll
@@ -70,7 +56,7 @@ __global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
   if (live) {
     if constexpr (MODE == MODE_SUMMARY) {
 #pragma unroll
-      for (int i = 0; i < NP; ++i) lds_node(sq, SB, i, t, acc[i]);
+      for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, acc[i]);
     } else {
       ElemGeo g;
       load_geo(a.geo, e, g);
@@ -84,7 +70,7 @@ __global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
 #pragma unroll 1
       for (int j = 0; j < NP; ++j) {
         double qj[4];
-        lds_node(sq, SB, j, t, qj);
+        load4(a.q + ((int64_t)j * K + e) * 4, qj);
         const Prim w = to_prim(qj, gm1);
         const double F[4] = {qj[1], qj[1] * w.u + w.p, qj[2] * w.u, (qj[3] + w.p) * w.u};
         const double G[4] = {qj[2], qj[1] * w.v, qj[2] * w.v + w.p, (qj[3] + w.p) * w.v};
@@ -129,12 +115,7 @@ __global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
             const int nf = fc & 3;
             const int pk = (fc & 4) ? (NFP - 1 - kk) : kk;
             const int pn = face_node_dyn(nf * NFP + pk);
-            const int64_t loc = (int64_t)nb - e0;
-            if (loc >= 0 && loc < ne) {
-              lds_node(sq, SB, pn, (int)loc, qp);
-            } else {
-              load4(a.q + ((int64_t)pn * K + nb) * 4, qp);
-            }
+            load4(a.q + ((int64_t)pn * K + nb) * 4, qp);
           } else if (kind == BC_SLIP) {
             const double mn = qj[1] * nx + qj[2] * ny;
             qp[0] = qj[0];
@@ -178,7 +159,7 @@ __global__ void __launch_bounds__(SB) k_stage(StageArgs a) {
           store4(o, acc[i]);
         } else {
           double qi[4];
-          lds_node(sq, SB, i, t, qi);
+          load4(a.q + ((int64_t)i * K + e) * 4, qi);
           double q0[4] = {0.0, 0.0, 0.0, 0.0};
           if constexpr (MODE != MODE_STAGE1) load4(a.q0 + ((int64_t)i * K + e) * 4, q0);
 #pragma unroll

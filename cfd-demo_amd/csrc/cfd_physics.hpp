@@ -27,25 +27,16 @@ constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.
 
 struct Prim {
   double rho, u, v, p;
+  double inv;  // 1 / rho
 };
 
 // conserved_to_primitive (euler.py:21-27).
 __device__ __forceinline__ Prim to_prim(const double q[4], double gm1) {
   Prim w;
   w.rho = q[0];
-  const double inv = 1.0 / q[0];
-  w.u = q[1] * inv;
-  w.v = q[2] * inv;
-  w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
-  return w;
-}
-
-// Exact-division variant used where a branch depends on the value.
-__device__ __forceinline__ Prim to_prim_div(const double q[4], double gm1) {
-  Prim w;
-  w.rho = q[0];
-  w.u = q[1] / q[0];
-  w.v = q[2] / q[0];
+  w.inv = 1.0 / q[0];
+  w.u = q[1] * w.inv;
+  w.v = q[2] * w.inv;
   w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
   return w;
 }
@@ -105,8 +96,8 @@ struct RoeAvg {
 // _roe_average (fluxes.py:40-51).
 __device__ __forceinline__ RoeAvg roe_average(const Side& m, const Side& p, double gm1) {
   RoeAvg a;
-  const double hm = (m.q[3] + m.w.p) / m.w.rho;
-  const double hp = (p.q[3] + p.w.p) / p.w.rho;
+  const double hm = (m.q[3] + m.w.p) * m.w.inv;
+  const double hp = (p.q[3] + p.w.p) * p.w.inv;
   const double wm = sqrt(m.w.rho);
   const double wp = sqrt(p.w.rho);
   const double inv = 1.0 / (wm + wp);
@@ -184,7 +175,7 @@ __device__ __forceinline__ void flux_hllc(const Side& m, const Side& p, double n
   star[0] = factor;
   star[1] = factor * (k.w.u + shift * nx);
   star[2] = factor * (k.w.v + shift * ny);
-  star[3] = factor * (k.q[3] / k.w.rho + shift * (s_star + k.w.p / safe_den(mass_k)));
+  star[3] = factor * (k.q[3] * k.w.inv + shift * (s_star + k.w.p / safe_den(mass_k)));
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[c] = k.f[c] + s_k * (star[c] - k.q[c]);
 }
