@@ -5,6 +5,7 @@ SSP-RK3 stages as hand-written HIP kernels behind the C ABI of libcfdhip.so.
 Exports the reference's public names (``src/cfd_demo/__init__.py:10-32``).
 """
 
+from . import batched  # noqa: F401  (registers the batched C-ABI entry points)
 from .boundary import FixedState, Periodic, SlipWall, Transmissive
 from .cases import SodShockTube, four_quadrant, isentropic_vortex, sod_shock_tube
 from .mesh import TriangleMesh, rectangle_mesh

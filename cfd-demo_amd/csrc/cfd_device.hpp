@@ -73,6 +73,19 @@ extern __constant__ double c_facew[NSLOT];
 extern __constant__ double c_p1mid[3 * NP];
 extern __constant__ double c_linb[NP * 3];
 
+// Batched solves: many meshes concatenated, each padded to a multiple of
+// 256 elements, so every workgroup (64, 128 or 256 elements) and every
+// 256-element limiter block lies inside one case.
+struct BatchView {
+  const int* lcase;        // case of each 256-element block
+  const int64_t* begin;    // first element of each case
+  const int* nreal;        // real (unpadded) elements of each case
+  const int* flux;         // flux id per case
+  const int* limiter;      // limiter id per case
+  const double* dt;        // dt of the current attempt per case (<= 0: case inactive)
+  int* wcount;             // per 256-element block: worklist length
+};
+
 struct StageArgs {
   const double* q;
   const double* q0;
@@ -85,6 +98,7 @@ struct StageArgs {
   int64_t K;
   double gamma;
   double dt;
+  BatchView bv;  // used only when FLUX == FLUX_DYNAMIC
 };
 
 struct LimitArgs {
@@ -104,6 +118,7 @@ struct LimitArgs {
   int limiter;
   int limit_slopes;
   int want_dt;
+  BatchView bv;  // bv.lcase != nullptr: batched mode, per-block worklists
 };
 
 struct DtArgs {

@@ -42,9 +42,30 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
   const int ne = (int)((K - e0) < SB ? (K - e0) : SB);
   const int t = threadIdx.x;
   const int64_t e = e0 + t;
-  const bool live = t < ne;
+  bool live = t < ne;
   const double gamma = a.gamma;
   const double gm1 = gamma - 1.0;
+  double dt = a.dt;
+  int flux_rt = FLUX;
+  if constexpr (FLUX == FLUX_DYNAMIC) {
+    const int cs = a.bv.lcase[e0 >> 8];
+    dt = a.bv.dt[cs];
+    flux_rt = a.bv.flux[cs];
+    live = live && (e - a.bv.begin[cs]) < a.bv.nreal[cs];
+    if (!(dt > 0.0)) {
+      // Inactive case: the last stage carries the step input over unchanged.
+      if constexpr (MODE == MODE_STAGE3) {
+        if (live)
+#pragma unroll
+          for (int i = 0; i < NP; ++i) {
+            double v[4];
+            load4(a.q0 + ((int64_t)i * K + e) * 4, v);
+            store4(a.out + ((int64_t)i * K + e) * 4, v);
+          }
+      }
+      return;
+    }
+  }
 
   double acc[NP][NV];
 #pragma unroll
@@ -131,7 +152,11 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
           Side p;
           make_side(p, qp, nx, ny, gamma, gm1);
           double fh[4];
-          interface_flux<FLUX>(m, p, nx, ny, gm1, fh);
+          if constexpr (FLUX == FLUX_DYNAMIC) {
+            interface_flux_rt(flux_rt, m, p, nx, ny, gm1, fh);
+          } else {
+            interface_flux<FLUX>(m, p, nx, ny, gm1, fh);
+          }
           double jump[4];
 #pragma unroll
           for (int c = 0; c < 4; ++c) jump[c] = fs * (m.f[c] - fh[c]);
@@ -151,7 +176,6 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
       }
 
       // Epilogue: write the RHS, or the SSP-RK3 combination of the stage.
-      const double dt = a.dt;
 #pragma unroll
       for (int i = 0; i < NP; ++i) {
         double* o = a.out + ((int64_t)i * K + e) * 4;
@@ -190,7 +214,15 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
   part_neutral(row);
-  if (e < a.K) {
+  bool live = e < a.K;
+  const bool batched = a.bv.lcase != nullptr;
+  if (batched && live) {
+    const int cs = a.bv.lcase[blk];
+    live = (a.bv.dt[cs] > 0.0) && (e - a.bv.begin[cs]) < a.bv.nreal[cs];
+    a.limiter = a.bv.limiter[cs];
+    a.limit_slopes = a.limiter != LIM_UNLIMITED;
+  }
+  if (live) {
     const double* sm = a.summ + e * SUMM_STRIDE;
     double avg[4];
     load4(sm, avg);
@@ -205,8 +237,14 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     const bool avg_ok = (avg[0] > 0.0) && (pbar > 0.0);
     row[COL_TROUBLED] = troubled ? 1.0 : 0.0;
     if (work) {
-      const int slot = atomicAdd(a.work_count, 1);
-      a.work[slot] = (int)e | (troubled ? (int)0x80000000u : 0);
+      const int entry = (int)e | (troubled ? (int)0x80000000u : 0);
+      if (batched) {
+        const int slot = atomicAdd(&a.bv.wcount[blk], 1);
+        a.work[(int64_t)blk * LBLOCK + slot] = entry;
+      } else {
+        const int slot = atomicAdd(a.work_count, 1);
+        a.work[slot] = entry;
+      }
       row[COL_ADM] = avg_ok ? 1.0 : 0.0;  // finiteness is checked by k_limit_fix
     } else {
       row[COL_RMIN] = sm[SUMM_RMIN];
@@ -223,45 +261,64 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
 // Limiter, pass 2: reconstruction and positivity for worklist elements. A
 // fixed grid strides over the worklist; every lane reaches the exit.
 // ---------------------------------------------------------------------------
+// Repair one worklist element in place; fold its report values into row.
+__device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, double row[PART_COLS]) {
+  const double gm1 = a.gamma - 1.0;
+  const int64_t e = entry & 0x7fffffff;
+  const bool troubled = entry < 0;
+  const int64_t K = a.K;
+  const double* sm = a.summ + e * SUMM_STRIDE;
+  double avg[4];
+  load4(sm, avg);
+  double q[NP][4];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, q[i]);
+  if (troubled) {
+    double sn[3][4];
+    detect_troubled(a, e, avg, sm, sn);
+    reconstruct_p1(q, avg, sn, a.limiter);
+  }
+  positivity(q, avg, a.gamma);
+  bool finite = true;
+  double rmin = INF, pmin = INF, lam = -INF;
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    store4(a.q + ((int64_t)i * K + e) * 4, q[i]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
+    rmin = nan_min(rmin, q[i][0]);
+    pmin = nan_min(pmin, pressure_of(q[i], gm1));
+    if (a.want_dt) lam = i == 0 ? wave_speed(q[i], a.gamma) : nan_max(lam, wave_speed(q[i], a.gamma));
+  }
+  row[COL_RMIN] = nan_min(row[COL_RMIN], rmin);
+  row[COL_PMIN] = nan_min(row[COL_PMIN], pmin);
+  if (!finite) row[COL_ADM] = 0.0;
+  if (a.want_dt) row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / lam);
+}
+
 __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
   double row[PART_COLS];
   part_neutral(row);
   const int n = *a.work_count;
-  const double gm1 = a.gamma - 1.0;
-  for (int idx = blockIdx.x * FBLOCK + threadIdx.x; idx < n; idx += gridDim.x * FBLOCK) {
-    const int entry = a.work[idx];
-    const int64_t e = entry & 0x7fffffff;
-    const bool troubled = entry < 0;
-    const int64_t K = a.K;
-    const double* sm = a.summ + e * SUMM_STRIDE;
-    double avg[4];
-    load4(sm, avg);
-    double q[NP][4];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, q[i]);
-    if (troubled) {
-      double sn[3][4];
-      detect_troubled(a, e, avg, sm, sn);
-      reconstruct_p1(q, avg, sn, a.limiter);
-    }
-    positivity(q, avg, a.gamma);
-    bool finite = true;
-    double rmin = INF, pmin = INF, lam = -INF;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      store4(a.q + ((int64_t)i * K + e) * 4, q[i]);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
-      rmin = nan_min(rmin, q[i][0]);
-      pmin = nan_min(pmin, pressure_of(q[i], gm1));
-      if (a.want_dt) lam = i == 0 ? wave_speed(q[i], a.gamma) : nan_max(lam, wave_speed(q[i], a.gamma));
-    }
-    row[COL_RMIN] = nan_min(row[COL_RMIN], rmin);
-    row[COL_PMIN] = nan_min(row[COL_PMIN], pmin);
-    if (!finite) row[COL_ADM] = 0.0;
-    if (a.want_dt) row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / lam);
-  }
+  for (int idx = blockIdx.x * FBLOCK + threadIdx.x; idx < n; idx += gridDim.x * FBLOCK)
+    repair_element(a, a.work[idx], row);
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)blockIdx.x * PART_STRIDE);
+}
+
+// Batched variant: block b repairs the worklist of 256-element block b, so
+// each case's report rows stay contiguous; it re-arms its list counter.
+__global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
+  double row[PART_COLS];
+  part_neutral(row);
+  const int b = blockIdx.x;
+  const int n = a.bv.wcount[b];
+  if (n > 0) {
+    const int cs = a.bv.lcase[b];
+    a.limiter = a.bv.limiter[cs];
+    for (int idx = threadIdx.x; idx < n; idx += FBLOCK) repair_element(a, a.work[(int64_t)b * LBLOCK + idx], row);
+  }
+  block_reduce_row<FBLOCK>(row, a.part + (int64_t)b * PART_STRIDE);
+  if (threadIdx.x == 0) a.bv.wcount[b] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -396,3 +453,4 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 }  // namespace cfd
 
 #include "cfd_capi.inc"
+#include "cfd_batched.inc"

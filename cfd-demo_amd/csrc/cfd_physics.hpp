@@ -18,7 +18,7 @@ constexpr int NFP = 5;       // nodes per face
 constexpr int NSLOT = 15;    // 3 faces x 5 face nodes
 constexpr int NV = 4;        // conserved variables
 
-enum FluxId : int { FLUX_RUSANOV = 0, FLUX_ROE = 1, FLUX_HLLC = 2 };
+enum FluxId : int { FLUX_RUSANOV = 0, FLUX_ROE = 1, FLUX_HLLC = 2, FLUX_DYNAMIC = 3 };
 enum LimiterId : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_VAN_LEER = 2, LIM_SUPERBEE = 3, LIM_UNLIMITED = 4 };
 enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE = 3 };
 
@@ -186,6 +186,18 @@ __device__ __forceinline__ void interface_flux(const Side& m, const Side& p, dou
   if constexpr (FLUX == FLUX_RUSANOV) {
     flux_rusanov(m, p, out);
   } else if constexpr (FLUX == FLUX_ROE) {
+    flux_roe(m, p, nx, ny, gm1, out);
+  } else {
+    flux_hllc(m, p, nx, ny, gm1, out);
+  }
+}
+
+// Flux chosen at run time (batched solves; the id is uniform per workgroup).
+__device__ __forceinline__ void interface_flux_rt(int id, const Side& m, const Side& p, double nx, double ny,
+                                                  double gm1, double out[4]) {
+  if (id == FLUX_RUSANOV) {
+    flux_rusanov(m, p, out);
+  } else if (id == FLUX_ROE) {
     flux_roe(m, p, nx, ny, gm1, out);
   } else {
     flux_hllc(m, p, nx, ny, gm1, out);

@@ -157,6 +157,52 @@ int cfd_timing_enable(CfdCtx* ctx, int on);
 int cfd_timing_read(CfdCtx* ctx, double* out4);
 int cfd_time_rhs(CfdCtx* ctx, int reps, double* avg_ms);
 
+/* Batched solves (no reference counterpart: the reference runs one case per
+ * EulerSolver). Many meshes are concatenated into one CfdMeshTables whose
+ * element count is a multiple of 256; case i owns elements
+ * [first, first + padded), padded a multiple of 256, of which the first
+ * `count` are real. Each case runs EulerSolver.run's loop (solver.py:167-216)
+ * with its own flux, limiter, CFL and final time; one batched SSP-RK3 step
+ * advances every active case by one of its own steps. */
+typedef struct {
+  int64_t first;
+  int64_t count;
+  int64_t padded;
+  int32_t flux_id;
+  int32_t limiter_id;
+  double cfl;
+  double final_time;
+  int32_t max_rejections;
+} CfdBatchCase;
+
+typedef struct {
+  double time;
+  int64_t steps;
+  int64_t rejected_steps;
+  double boundary_flux_integral[4];
+  double min_density;
+  double min_pressure;
+  int64_t max_troubled_cells;
+  int32_t status; /* 0 running, 1 done, 2 bad initial state, 3 invalid dt, 4 rejected too often */
+  double fail_time;
+  double fail_dt;
+} CfdBatchCaseResult;
+
+typedef struct CfdBatch CfdBatch;
+
+int cfd_batch_create(const CfdMeshTables* mesh, int ncase, const CfdBatchCase* cases, double gamma, CfdBatch** out);
+int cfd_batch_destroy(CfdBatch* batch);
+/* Upload the concatenated (15, K, 4) state (padding elements hold an
+ * admissible state of their case), check it and compute each first dt. */
+int cfd_batch_begin(CfdBatch* batch, const double* q);
+/* Up to nsteps batched steps (use_graph: replay a captured hipGraph). */
+int cfd_batch_advance(CfdBatch* batch, int64_t nsteps, int use_graph, int64_t* active_left);
+int cfd_batch_download(CfdBatch* batch, double* q);
+int cfd_batch_results(CfdBatch* batch, CfdBatchCaseResult* results);
+/* Autotuner: time the stage kernel at 64/128/256 elements per workgroup on
+ * the current state and keep the fastest (ms_out[3] may be NULL). */
+int cfd_batch_autotune(CfdBatch* batch, int reps, int* chosen, double* ms_out);
+
 /* Pointwise interface flux (fluxes.py:28-143) on n face states: qm, qp (n,4),
  * nx, ny (n), out (n,4); host arrays. */
 int cfd_flux_pointwise(int flux_id, int64_t n, const double* qm, const double* qp, const double* nx,
