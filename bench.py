@@ -74,6 +74,44 @@ def cpu_baseline(n: int, steps: int) -> dict:
                       f"triangles), {el:.1f} s"}
 
 
+def batched_bench(n_cases: int, steps: int, seed: int = 0) -> dict:
+    """Seeded random shock tubes: one batched launch per stage for all cases
+    (hipGraph replay) against the plain per-case loop of EulerSolver.run, both
+    over the same number of steps per case."""
+    from cfd_demo_amd.batched import BatchedSolver, random_shock_tube_specs, shock_tube_case
+
+    cases = [shock_tube_case(s) for s in random_shock_tube_specs(n_cases, seed=seed)]
+    dof = sum(60 * c.solver.mesh.num_elements for c in cases)
+    batch = BatchedSolver(cases)
+    batch.begin()
+    sb, ms = batch.autotune(reps=3)
+    batch.advance(2)  # warm-up (includes graph capture)
+    batch.begin()
+    t0 = time.perf_counter()
+    left = batch.advance(steps)
+    t_batch = time.perf_counter() - t0
+    done_steps = [r.steps for r in batch.results()]
+    batch.close()
+    log(f"batched: {n_cases} cases, {dof} DOF, stage block {sb} (ms {ms.round(3).tolist()}), "
+        f"{steps} steps in {t_batch:.3f}s, active left {left}")
+    for c in cases:
+        c.solver._context()
+    t0 = time.perf_counter()
+    loop_steps = []
+    for c in cases:
+        r = c.solver.run(c.initial_state, c.final_time, max_steps=steps)
+        loop_steps.append(r.steps)
+    t_loop = time.perf_counter() - t0
+    for c in cases:
+        c.solver.close()
+    work = sum(3 * 60 * c.solver.mesh.num_elements * s for c, s in zip(cases, done_steps))
+    work_loop = sum(3 * 60 * c.solver.mesh.num_elements * s for c, s in zip(cases, loop_steps))
+    log(f"per-case loop: {t_loop:.3f}s")
+    return {"cases": n_cases, "seed": seed, "dof": dof, "steps_per_case": steps, "stage_block": sb,
+            "batched_dof_stage_per_s": work / t_batch, "loop_dof_stage_per_s": work_loop / t_loop,
+            "speedup_vs_loop": (work / t_batch) / (work_loop / t_loop)}
+
+
 def traffic_from_profile(kernel_prefix: str):
     p = ROOT / "profiles" / "stage_kernel_c2_traffic.json"
     if not p.exists():
@@ -94,6 +132,9 @@ def main() -> int:
     ap.add_argument("--cpu-n", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-cases", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=30)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -194,6 +235,8 @@ def main() -> int:
         "limiter_mean_launch_ms": limit_ms,
         "cpu_baseline": None,
     }
+    if not args.no_batched:
+        line["batched"] = batched_bench(args.batched_cases, args.batched_steps)
     if not args.no_cpu:
         log(f"cpu baseline: oracle at {args.cpu_n}x{args.cpu_n}, {args.cpu_steps} steps")
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
