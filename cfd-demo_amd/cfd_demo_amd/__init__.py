@@ -6,13 +6,15 @@ Exports the reference's public names (``src/cfd_demo/__init__.py:10-32``).
 """
 
 from . import batched  # noqa: F401  (registers the batched C-ABI entry points)
+from .adjoint import DiscreteAdjoint, evolve
 from .boundary import FixedState, Periodic, SlipWall, Transmissive
-from .cases import SodShockTube, four_quadrant, isentropic_vortex, sod_shock_tube
+from .cases import SodShockTube, four_quadrant, isentropic_vortex, sod_shock_tube, vortex_bumps
 from .mesh import TriangleMesh, rectangle_mesh
 from .physics import INTERFACE_FLUXES, LIMITER_FUNCTIONS, get_interface_flux, get_limiter_function
 from .solver import EulerSolver, RunResult, SolverConfig, SolverError, StepReport, apply_limiters
 
 __all__ = [
+    "DiscreteAdjoint",
     "INTERFACE_FLUXES",
     "LIMITER_FUNCTIONS",
     "EulerSolver",
@@ -27,10 +29,12 @@ __all__ = [
     "Transmissive",
     "TriangleMesh",
     "apply_limiters",
+    "evolve",
     "four_quadrant",
     "get_interface_flux",
     "get_limiter_function",
     "isentropic_vortex",
     "rectangle_mesh",
     "sod_shock_tube",
+    "vortex_bumps",
 ]

@@ -53,6 +53,13 @@ class CfdRunResult(C.Structure):
                 ("fail_dt", C.c_double)]
 
 
+class CfdAdjointStats(C.Structure):
+    _fields_ = [("num_steps", C.c_int64), ("checkpoints", C.c_int64), ("tape_records", C.c_int64),
+                ("tape_peak", C.c_int64), ("tape_regrows", C.c_int64), ("positivity_scaled", C.c_int64),
+                ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("recompute_ms", C.c_double),
+                ("device_bytes", C.c_int64)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "cfd_last_error": (C.c_char_p, []),
@@ -78,6 +85,14 @@ SIGNATURES = {
     "cfd_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "cfd_timing_read": (C.c_int, [C.c_void_p, _dp]),
     "cfd_time_rhs": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "cfd_rhs_vjp": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "cfd_limit_vjp": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "cfd_adjoint_create": (C.c_int, [C.c_void_p, C.c_int64, _dp, C.c_int, C.POINTER(C.c_void_p)]),
+    "cfd_adjoint_destroy": (C.c_int, [C.c_void_p]),
+    "cfd_adjoint_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cfd_adjoint_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cfd_adjoint_time_steps": (C.c_int, [C.c_void_p, _dp]),
+    "cfd_adjoint_stats": (C.c_int, [C.c_void_p, C.POINTER(CfdAdjointStats)]),
     "cfd_flux_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double, _dp]),
     "cfd_limiter_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp]),
 }

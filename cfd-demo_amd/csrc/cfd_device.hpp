@@ -119,7 +119,22 @@ struct LimitArgs {
   int limit_slopes;
   int want_dt;
   BatchView bv;  // bv.lcase != nullptr: batched mode, per-block worklists
+  // Adjoint tape (nullptr: off). Worklist slot s of this pass is recorded at
+  // record *tape_head + s when that is below tape_cap.
+  double* tape = nullptr;
+  const int64_t* tape_head = nullptr;
+  int64_t tape_cap = 0;
 };
+
+// Tape record of one limiter worklist element (TAPE_STRIDE doubles):
+// element id, troubled (0/1), positivity scale theta, the scaled neighbour
+// deviations sn[3][4] and the pre-limit nodal values [NP][4].
+constexpr int TAPE_STRIDE = 80;
+constexpr int TAPE_E = 0;
+constexpr int TAPE_TROUBLED = 1;
+constexpr int TAPE_THETA = 2;
+constexpr int TAPE_SN = 4;
+constexpr int TAPE_Q = 16;
 
 struct DtArgs {
   const double* q;
@@ -414,19 +429,22 @@ __device__ __forceinline__ void reconstruct_p1(double q[NP][4], const double avg
     }
 }
 
-// Zhang-Shu positivity scaling (limiting.py:166-200), per element.
-__device__ __forceinline__ void positivity(double q[NP][4], const double avg[4], double gamma) {
+// Zhang-Shu positivity scaling (limiting.py:166-200), per element. Returns the
+// overall scale theta of q - avg that was applied (1 when nothing changed).
+__device__ __forceinline__ double positivity(double q[NP][4], const double avg[4], double gamma) {
 #pragma clang fp contract(off)
   const double gm1 = gamma - 1.0;
   const double pbar = pressure_of(avg, gm1);
   const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
   double rmin = INF;
+  double scale = 1.0;
 #pragma unroll
   for (int i = 0; i < NP; ++i) rmin = nan_min(rmin, q[i][0]);
   if (rmin < eps) {
     const double span = avg[0] - rmin;
     double theta = (avg[0] - eps) / span;
     theta = fmin(fmax(theta, 0.0), 1.0);
+    scale = theta;
 #pragma unroll
     for (int i = 0; i < NP; ++i)
 #pragma unroll
@@ -439,7 +457,7 @@ __device__ __forceinline__ void positivity(double q[NP][4], const double avg[4],
     need[i] = pressure_of(q[i], gm1) < eps;
     any = any || need[i];
   }
-  if (!any) return;
+  if (!any) return scale;
   double theta = 1.0;
   for (int i = 0; i < NP; ++i) {
     if (!need[i]) continue;
@@ -463,7 +481,9 @@ __device__ __forceinline__ void positivity(double q[NP][4], const double avg[4],
     for (int i = 0; i < NP; ++i)
 #pragma unroll
       for (int c = 0; c < 4; ++c) q[i][c] = avg[c] + theta * (q[i][c] - avg[c]);
+    scale = scale * theta;
   }
+  return scale;
 }
 
 }  // namespace cfd

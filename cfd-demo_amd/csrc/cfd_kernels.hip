@@ -262,7 +262,9 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
 // fixed grid strides over the worklist; every lane reaches the exit.
 // ---------------------------------------------------------------------------
 // Repair one worklist element in place; fold its report values into row.
-__device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, double row[PART_COLS]) {
+// With a tape, worklist slot `slot` also records the pre-limit values, the
+// neighbour deviations and the positivity scale for the adjoint.
+__device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, int64_t slot, double row[PART_COLS]) {
   const double gm1 = a.gamma - 1.0;
   const int64_t e = entry & 0x7fffffff;
   const bool troubled = entry < 0;
@@ -273,12 +275,26 @@ __device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, do
   double q[NP][4];
 #pragma unroll
   for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, q[i]);
-  if (troubled) {
-    double sn[3][4];
-    detect_troubled(a, e, avg, sm, sn);
-    reconstruct_p1(q, avg, sn, a.limiter);
+  double sn[3][4];
+  if (troubled) detect_troubled(a, e, avg, sm, sn);
+  double* rec = nullptr;
+  if (a.tape) {
+    const int64_t pos = *a.tape_head + slot;
+    if (pos < a.tape_cap) rec = a.tape + pos * TAPE_STRIDE;
   }
-  positivity(q, avg, a.gamma);
+  if (rec) {
+    rec[TAPE_E] = (double)e;
+    rec[TAPE_TROUBLED] = troubled ? 1.0 : 0.0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) rec[TAPE_SN + 4 * f + c] = troubled ? sn[f][c] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) store4(rec + TAPE_Q + 4 * i, q[i]);
+  }
+  if (troubled) reconstruct_p1(q, avg, sn, a.limiter);
+  const double theta = positivity(q, avg, a.gamma);
+  if (rec) rec[TAPE_THETA] = theta;
   bool finite = true;
   double rmin = INF, pmin = INF, lam = -INF;
 #pragma unroll
@@ -301,7 +317,7 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
   part_neutral(row);
   const int n = *a.work_count;
   for (int idx = blockIdx.x * FBLOCK + threadIdx.x; idx < n; idx += gridDim.x * FBLOCK)
-    repair_element(a, a.work[idx], row);
+    repair_element(a, a.work[idx], idx, row);
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)blockIdx.x * PART_STRIDE);
 }
 
@@ -315,7 +331,7 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
   if (n > 0) {
     const int cs = a.bv.lcase[b];
     a.limiter = a.bv.limiter[cs];
-    for (int idx = threadIdx.x; idx < n; idx += FBLOCK) repair_element(a, a.work[(int64_t)b * LBLOCK + idx], row);
+    for (int idx = threadIdx.x; idx < n; idx += FBLOCK) repair_element(a, a.work[(int64_t)b * LBLOCK + idx], -1, row);
   }
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)b * PART_STRIDE);
   if (threadIdx.x == 0) a.bv.wcount[b] = 0;
@@ -452,5 +468,7 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 
 }  // namespace cfd
 
+#include "cfd_adjoint_kernels.inc"
 #include "cfd_capi.inc"
 #include "cfd_batched.inc"
+#include "cfd_adjoint.inc"

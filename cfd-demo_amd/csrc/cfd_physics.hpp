@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "cfd_dual.hpp"
+
 namespace cfd {
 
 constexpr int NP = 15;       // nodes per degree-4 triangle
@@ -25,14 +27,17 @@ enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE 
 constexpr double ROE_ENTROPY_FIX = 0.1;          // fluxes.py:21
 constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.py:24-25
 
-struct Prim {
-  double rho, u, v, p;
-  double inv;  // 1 / rho
+template <class T>
+struct PrimT {
+  T rho, u, v, p;
+  T inv;  // 1 / rho
 };
+using Prim = PrimT<double>;
 
 // conserved_to_primitive (euler.py:21-27).
-__device__ __forceinline__ Prim to_prim(const double q[4], double gm1) {
-  Prim w;
+template <class T>
+__device__ __forceinline__ PrimT<T> to_prim(const T q[4], double gm1) {
+  PrimT<T> w;
   w.rho = q[0];
   w.inv = 1.0 / q[0];
   w.u = q[1] * w.inv;
@@ -50,27 +55,33 @@ __device__ __forceinline__ double pressure_of(const double q[4], double gm1) {
 }
 
 // normal_flux (euler.py:49-61).
-__device__ __forceinline__ void normal_flux(const double q[4], const Prim& w, double nx, double ny,
-                                            double out[4]) {
-  const double un = w.u * nx + w.v * ny;
+template <class T>
+__device__ __forceinline__ void normal_flux(const T q[4], const PrimT<T>& w, double nx, double ny, T out[4]) {
+  const T un = w.u * nx + w.v * ny;
   out[0] = w.rho * un;
   out[1] = q[1] * un + w.p * nx;
   out[2] = q[2] * un + w.p * ny;
   out[3] = (q[3] + w.p) * un;
 }
 
-__device__ __forceinline__ double safe_den(double x) { return x == 0.0 ? TINY : x; }
+template <class T>
+__device__ __forceinline__ T safe_den(const T& x) {
+  return x == 0.0 ? T(TINY) : x;
+}
 
 // Minus-side data computed once per face node and reused by every flux.
-struct Side {
-  double q[4];
-  Prim w;
-  double c;   // sound speed
-  double un;  // normal velocity
-  double f[4];  // F(q) . n
+template <class T>
+struct SideT {
+  T q[4];
+  PrimT<T> w;
+  T c;     // sound speed
+  T un;    // normal velocity
+  T f[4];  // F(q) . n
 };
+using Side = SideT<double>;
 
-__device__ __forceinline__ void make_side(Side& s, const double q[4], double nx, double ny, double gamma,
+template <class T>
+__device__ __forceinline__ void make_side(SideT<T>& s, const T q[4], double nx, double ny, double gamma,
                                           double gm1) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) s.q[c] = q[c];
@@ -81,26 +92,29 @@ __device__ __forceinline__ void make_side(Side& s, const double q[4], double nx,
 }
 
 // rusanov (fluxes.py:28-37).
-__device__ __forceinline__ void flux_rusanov(const Side& m, const Side& p, double out[4]) {
-  const double sm = fabs(m.un) + m.c;
-  const double sp = fabs(p.un) + p.c;
-  const double alpha = fmax(sm, sp);
+template <class T>
+__device__ __forceinline__ void flux_rusanov(const SideT<T>& m, const SideT<T>& p, T out[4]) {
+  const T sm = fabs(m.un) + m.c;
+  const T sp = fabs(p.un) + p.c;
+  const T alpha = fmax(sm, sp);
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[c] = 0.5 * (m.f[c] + p.f[c]) - 0.5 * alpha * (p.q[c] - m.q[c]);
 }
 
-struct RoeAvg {
-  double rho, u, v, h, c;
+template <class T>
+struct RoeAvgT {
+  T rho, u, v, h, c;
 };
 
 // _roe_average (fluxes.py:40-51).
-__device__ __forceinline__ RoeAvg roe_average(const Side& m, const Side& p, double gm1) {
-  RoeAvg a;
-  const double hm = (m.q[3] + m.w.p) * m.w.inv;
-  const double hp = (p.q[3] + p.w.p) * p.w.inv;
-  const double wm = sqrt(m.w.rho);
-  const double wp = sqrt(p.w.rho);
-  const double inv = 1.0 / (wm + wp);
+template <class T>
+__device__ __forceinline__ RoeAvgT<T> roe_average(const SideT<T>& m, const SideT<T>& p, double gm1) {
+  RoeAvgT<T> a;
+  const T hm = (m.q[3] + m.w.p) * m.w.inv;
+  const T hp = (p.q[3] + p.w.p) * p.w.inv;
+  const T wm = sqrt(m.w.rho);
+  const T wp = sqrt(p.w.rho);
+  const T inv = 1.0 / (wm + wp);
   a.u = (wm * m.w.u + wp * p.w.u) * inv;
   a.v = (wm * m.w.v + wp * p.w.v) * inv;
   a.h = (wm * hm + wp * hp) * inv;
@@ -110,34 +124,35 @@ __device__ __forceinline__ RoeAvg roe_average(const Side& m, const Side& p, doub
 }
 
 // roe (fluxes.py:54-98).
-__device__ __forceinline__ void flux_roe(const Side& m, const Side& p, double nx, double ny, double gm1,
-                                         double out[4]) {
-  const RoeAvg a = roe_average(m, p, gm1);
-  const double un = a.u * nx + a.v * ny;
-  const double ut = -a.u * ny + a.v * nx;
-  const double d_rho = p.w.rho - m.w.rho;
-  const double d_p = p.w.p - m.w.p;
-  const double d_un = p.un - m.un;
-  const double d_ut = (-p.w.u * ny + p.w.v * nx) - (-m.w.u * ny + m.w.v * nx);
-  const double c2 = a.c * a.c;
-  const double inv2c2 = 1.0 / (2.0 * c2);
-  const double a_minus = (d_p - a.rho * a.c * d_un) * inv2c2;
-  const double a_entropy = d_rho - d_p / c2;
-  const double a_shear = a.rho * d_ut;
-  const double a_plus = (d_p + a.rho * a.c * d_un) * inv2c2;
-  const double delta = ROE_ENTROPY_FIX * (fabs(un) + a.c);
-  auto fix = [delta](double lam) {
-    const double mag = fabs(lam);
+template <class T>
+__device__ __forceinline__ void flux_roe(const SideT<T>& m, const SideT<T>& p, double nx, double ny, double gm1,
+                                         T out[4]) {
+  const RoeAvgT<T> a = roe_average(m, p, gm1);
+  const T un = a.u * nx + a.v * ny;
+  const T ut = -a.u * ny + a.v * nx;
+  const T d_rho = p.w.rho - m.w.rho;
+  const T d_p = p.w.p - m.w.p;
+  const T d_un = p.un - m.un;
+  const T d_ut = (-p.w.u * ny + p.w.v * nx) - (-m.w.u * ny + m.w.v * nx);
+  const T c2 = a.c * a.c;
+  const T inv2c2 = 1.0 / (2.0 * c2);
+  const T a_minus = (d_p - a.rho * a.c * d_un) * inv2c2;
+  const T a_entropy = d_rho - d_p / c2;
+  const T a_shear = a.rho * d_ut;
+  const T a_plus = (d_p + a.rho * a.c * d_un) * inv2c2;
+  const T delta = ROE_ENTROPY_FIX * (fabs(un) + a.c);
+  auto fix = [delta](const T& lam) {
+    const T mag = fabs(lam);
     return mag < delta ? (lam * lam + delta * delta) / (2.0 * delta) : mag;
   };
-  const double lm = fix(un - a.c) * a_minus;
-  const double lp = fix(un + a.c) * a_plus;
-  const double le = fabs(un) * a_entropy;
-  const double ls = fabs(un) * a_shear;
-  const double d0 = lm + le + lp;
-  const double d1 = lm * (a.u - a.c * nx) + le * a.u + ls * (-ny) + lp * (a.u + a.c * nx);
-  const double d2 = lm * (a.v - a.c * ny) + le * a.v + ls * nx + lp * (a.v + a.c * ny);
-  const double d3 = lm * (a.h - a.c * un) + le * (0.5 * (a.u * a.u + a.v * a.v)) + ls * ut + lp * (a.h + a.c * un);
+  const T lm = fix(un - a.c) * a_minus;
+  const T lp = fix(un + a.c) * a_plus;
+  const T le = fabs(un) * a_entropy;
+  const T ls = fabs(un) * a_shear;
+  const T d0 = lm + le + lp;
+  const T d1 = lm * (a.u - a.c * nx) + le * a.u + ls * (-ny) + lp * (a.u + a.c * nx);
+  const T d2 = lm * (a.v - a.c * ny) + le * a.v + ls * nx + lp * (a.v + a.c * ny);
+  const T d3 = lm * (a.h - a.c * un) + le * (0.5 * (a.u * a.u + a.v * a.v)) + ls * ut + lp * (a.h + a.c * un);
   out[0] = 0.5 * (m.f[0] + p.f[0]) - 0.5 * d0;
   out[1] = 0.5 * (m.f[1] + p.f[1]) - 0.5 * d1;
   out[2] = 0.5 * (m.f[2] + p.f[2]) - 0.5 * d2;
@@ -146,15 +161,16 @@ __device__ __forceinline__ void flux_roe(const Side& m, const Side& p, double nx
 
 // hllc (fluxes.py:101-143). Only the selected branch's star state is built;
 // the branch order and the >=/> tests are the reference's.
-__device__ __forceinline__ void flux_hllc(const Side& m, const Side& p, double nx, double ny, double gm1,
-                                          double out[4]) {
-  const RoeAvg a = roe_average(m, p, gm1);
-  const double un = a.u * nx + a.v * ny;
-  const double s_m = fmin(m.un - m.c, un - a.c);
-  const double s_p = fmax(p.un + p.c, un + a.c);
-  const double mass_m = m.w.rho * (s_m - m.un);
-  const double mass_p = p.w.rho * (s_p - p.un);
-  const double s_star = (p.w.p - m.w.p + mass_m * m.un - mass_p * p.un) / safe_den(mass_m - mass_p);
+template <class T>
+__device__ __forceinline__ void flux_hllc(const SideT<T>& m, const SideT<T>& p, double nx, double ny, double gm1,
+                                          T out[4]) {
+  const RoeAvgT<T> a = roe_average(m, p, gm1);
+  const T un = a.u * nx + a.v * ny;
+  const T s_m = fmin(m.un - m.c, un - a.c);
+  const T s_p = fmax(p.un + p.c, un + a.c);
+  const T mass_m = m.w.rho * (s_m - m.un);
+  const T mass_p = p.w.rho * (s_p - p.un);
+  const T s_star = (p.w.p - m.w.p + mass_m * m.un - mass_p * p.un) / safe_den(mass_m - mass_p);
   if (s_m >= 0.0) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = m.f[c];
@@ -166,12 +182,12 @@ __device__ __forceinline__ void flux_hllc(const Side& m, const Side& p, double n
     for (int c = 0; c < 4; ++c) out[c] = p.f[c];
     return;
   }
-  const Side& k = left ? m : p;
-  const double s_k = left ? s_m : s_p;
-  const double mass_k = left ? mass_m : mass_p;
-  const double factor = mass_k / safe_den(s_k - s_star);
-  const double shift = s_star - k.un;
-  double star[4];
+  const SideT<T>& k = left ? m : p;
+  const T s_k = left ? s_m : s_p;
+  const T mass_k = left ? mass_m : mass_p;
+  const T factor = mass_k / safe_den(s_k - s_star);
+  const T shift = s_star - k.un;
+  T star[4];
   star[0] = factor;
   star[1] = factor * (k.w.u + shift * nx);
   star[2] = factor * (k.w.v + shift * ny);
@@ -180,9 +196,9 @@ __device__ __forceinline__ void flux_hllc(const Side& m, const Side& p, double n
   for (int c = 0; c < 4; ++c) out[c] = k.f[c] + s_k * (star[c] - k.q[c]);
 }
 
-template <int FLUX>
-__device__ __forceinline__ void interface_flux(const Side& m, const Side& p, double nx, double ny, double gm1,
-                                               double out[4]) {
+template <int FLUX, class T>
+__device__ __forceinline__ void interface_flux(const SideT<T>& m, const SideT<T>& p, double nx, double ny, double gm1,
+                                               T out[4]) {
   if constexpr (FLUX == FLUX_RUSANOV) {
     flux_rusanov(m, p, out);
   } else if constexpr (FLUX == FLUX_ROE) {
@@ -193,8 +209,9 @@ __device__ __forceinline__ void interface_flux(const Side& m, const Side& p, dou
 }
 
 // Flux chosen at run time (batched solves; the id is uniform per workgroup).
-__device__ __forceinline__ void interface_flux_rt(int id, const Side& m, const Side& p, double nx, double ny,
-                                                  double gm1, double out[4]) {
+template <class T>
+__device__ __forceinline__ void interface_flux_rt(int id, const SideT<T>& m, const SideT<T>& p, double nx, double ny,
+                                                  double gm1, T out[4]) {
   if (id == FLUX_RUSANOV) {
     flux_rusanov(m, p, out);
   } else if (id == FLUX_ROE) {
@@ -205,7 +222,8 @@ __device__ __forceinline__ void interface_flux_rt(int id, const Side& m, const S
 }
 
 // Limiter functions phi(r) (limiters.py:16-39), selected at run time.
-__device__ __forceinline__ double limiter_phi(int id, double r) {
+template <class T>
+__device__ __forceinline__ T limiter_phi(int id, const T& r) {
   switch (id) {
     case LIM_MINMOD:
       return fmax(0.0, fmin(1.0, r));
@@ -216,7 +234,7 @@ __device__ __forceinline__ double limiter_phi(int id, double r) {
     case LIM_SUPERBEE:
       return fmax(0.0, fmax(fmin(2.0 * r, 1.0), fmin(r, 2.0)));
     default:
-      return 1.0;
+      return T(1.0);
   }
 }
 

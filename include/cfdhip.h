@@ -203,6 +203,46 @@ int cfd_batch_results(CfdBatch* batch, CfdBatchCaseResult* results);
  * the current state and keep the fastest (ms_out[3] may be NULL). */
 int cfd_batch_autotune(CfdBatch* batch, int reps, int* chosen, double* ms_out);
 
+/* Discrete adjoint (no reference counterpart: the reference differentiates
+ * only its pointwise fluxes and limiters with jax.grad,
+ * tests/unit/test_physics.py:164-182,224-229). Derivatives are those of the
+ * SSP-RK3 map (time_integration.py:44-84) with each limiter pass's troubled
+ * flags and positivity scale held at their forward values and dt fixed.
+ *
+ * cfd_rhs_vjp:   out = (dL/dq)^T mu for the DG RHS L of dg.py:140-165.
+ * cfd_limit_vjp: out = (dLim/dq)^T lam for apply_limiters
+ *                (limiting.py:203-216) at q; host arrays (15, K, 4). */
+int cfd_rhs_vjp(CfdCtx* ctx, const double* q, const double* mu, double* out);
+int cfd_limit_vjp(CfdCtx* ctx, const double* q, const double* lam, double* out);
+
+/* A discrete-adjoint run: num_steps SSP-RK3 steps with the time steps dts
+ * (NULL: the first forward records the CFL steps, cfl of the context, and
+ * later calls reuse them). The forward keeps a checkpoint every
+ * checkpoint_every steps; the backward recomputes each segment from its
+ * checkpoint, storing the stage states and a tape of the limiter's repaired
+ * elements, then runs the adjoint steps in reverse. State pointers are
+ * device pointers to (15, K, 4) float64; both calls synchronise. */
+typedef struct CfdAdjoint CfdAdjoint;
+typedef struct {
+  int64_t num_steps;
+  int64_t checkpoints;
+  int64_t tape_records;      /* records the tape holds */
+  int64_t tape_peak;         /* most records one segment used */
+  int64_t tape_regrows;      /* segments recomputed after growing the tape */
+  int64_t positivity_scaled; /* tape records whose positivity scale was < 1 (last backward) */
+  double forward_ms;         /* last forward, host wall clock */
+  double backward_ms;        /* last backward, host wall clock */
+  double recompute_ms;       /* part of backward_ms spent recomputing segments */
+  int64_t device_bytes;      /* device memory held by the run */
+} CfdAdjointStats;
+
+int cfd_adjoint_create(CfdCtx* ctx, int64_t num_steps, const double* dts, int checkpoint_every, CfdAdjoint** out);
+int cfd_adjoint_destroy(CfdAdjoint* adj);
+int cfd_adjoint_forward(CfdAdjoint* adj, const double* q0_dev, double* qn_dev);
+int cfd_adjoint_backward(CfdAdjoint* adj, const double* gn_dev, double* g0_dev);
+int cfd_adjoint_time_steps(CfdAdjoint* adj, double* dts);
+int cfd_adjoint_stats(CfdAdjoint* adj, CfdAdjointStats* stats);
+
 /* Pointwise interface flux (fluxes.py:28-143) on n face states: qm, qp (n,4),
  * nx, ny (n), out (n,4); host arrays. */
 int cfd_flux_pointwise(int flux_id, int64_t n, const double* qm, const double* qp, const double* nx,

@@ -129,8 +129,11 @@ def troubled_cell_limit(q, avg, ops, phi):
     return np.where(troubled[None, :, None], linear, q), troubled
 
 
-def positivity_limit(q, avg, gamma):
-    """Zhang-Shu scaling (``limiting.py:166-200``), including the global cond."""
+def positivity_limit(q, avg, gamma, return_theta=False):
+    """Zhang-Shu scaling (``limiting.py:166-200``), including the global cond.
+
+    ``return_theta`` also returns the overall scale (K,) applied to q - avg
+    (1 where nothing changed), the value the adjoint holds fixed."""
     rbar = avg[:, 0]
     pbar = pressure(avg, gamma)
     eps = np.minimum(POSITIVITY_FLOOR, np.minimum(rbar, pbar))
@@ -141,9 +144,10 @@ def positivity_limit(q, avg, gamma):
     with np.errstate(divide="ignore", invalid="ignore"):
         theta = np.clip(np.where(need_r, (rbar - eps) / span, 1.0), 0.0, 1.0)
     q = np.where(need_r[None, :, None], avg[None] + theta[None, :, None] * dev, q)
+    theta_r = theta
     need_p = pressure(q, gamma) < eps[None]
     if not np.any(need_p):
-        return q
+        return (q, theta_r) if return_theta else q
     dev = q - avg[None]
     lo = np.zeros(q.shape[:2])
     hi = np.ones(q.shape[:2])
@@ -153,7 +157,8 @@ def positivity_limit(q, avg, gamma):
         lo, hi = np.where(ok, mid, lo), np.where(ok, hi, mid)
     theta = np.min(np.where(need_p, lo, 1.0), axis=0)
     lim = avg[None] + theta[None, :, None] * dev
-    return np.where((theta < 1.0)[None, :, None], lim, q)
+    out = np.where((theta < 1.0)[None, :, None], lim, q)
+    return (out, theta_r * np.where(theta < 1.0, theta, 1.0)) if return_theta else out
 
 
 def apply_limiters(q, ops, phi, gamma, limit_slopes, return_flags=False):
