@@ -112,6 +112,63 @@ def batched_bench(n_cases: int, steps: int, seed: int = 0) -> dict:
             "speedup_vs_loop": (work / t_batch) / (work_loop / t_loop)}
 
 
+def adjoint_bench(n: int, batch: int, steps: int, checkpoint_every: int, fd_dirs: int, seed: int = 0) -> dict:
+    """Vortex plus 8 seeded Gaussian bumps (field b uses seed + b), periodic
+    [0,10]^2 on n x n rectangles, Rusanov, unlimited, CFL 0.1; J = sum over
+    the batch of int_[6,8]^2 p^2 by nodal quadrature. Times the checkpointed
+    forward (frozen CFL steps) and the backward (segment recomputation plus
+    adjoint kernels) through the torch.autograd wrapper's two passes, and
+    checks fd_dirs random directions by central differences."""
+    import torch
+
+    from cfd_demo_amd.adjoint import DiscreteAdjoint, pressure_squared_objective, window_weights
+    from cfd_demo_amd.cases import vortex_bumps, vortex_bumps_state
+
+    case = vortex_bumps(n, flux="rusanov", limiter="unlimited", cfl=0.1)
+    s = case.solver
+    x, y = s.node_coordinates()
+    q0 = torch.from_numpy(np.stack([vortex_bumps_state(x, y, seed=seed + b) for b in range(batch)])).cuda()
+    w = torch.from_numpy(window_weights(s)).cuda()
+    gamma = s.config.gamma
+    run = DiscreteAdjoint(s, steps, batch=batch, checkpoint_every=checkpoint_every)
+    t0 = time.perf_counter()
+    run.forward(q0)  # records the CFL time steps (one host sync per step)
+    t_record = time.perf_counter() - t0
+    xg = q0.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    qn = run.forward(xg.detach())
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    qn.requires_grad_(True)
+    J = pressure_squared_objective(qn, w, gamma)
+    (gn,) = torch.autograd.grad(J, qn)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g0 = run.backward(gn)
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    st = run.stats()
+    rng = np.random.default_rng(seed + 1000)
+    worst = 0.0
+    with torch.no_grad():
+        for _ in range(fd_dirs):
+            d = torch.from_numpy(rng.standard_normal(tuple(q0.shape))).cuda() * q0.abs()
+            h = 1e-6
+            jp = pressure_squared_objective(run.forward(q0 + h * d), w, gamma).item()
+            jm = pressure_squared_objective(run.forward(q0 - h * d), w, gamma).item()
+            fd = (jp - jm) / (2 * h)
+            worst = max(worst, abs(float((g0 * d).sum()) - fd) / abs(fd))
+    run.close()
+    dof = 60 * s.mesh.num_elements * batch
+    log(f"adjoint: {batch} x {s.mesh.num_elements} elements, {steps} steps, forward {t_fwd:.3f}s, backward "
+        f"{t_bwd:.3f}s (recompute {st['recompute_ms'] / 1e3:.3f}s), ratio {t_bwd / t_fwd:.2f}, fd rel {worst:.2e}")
+    return {"fields": batch, "elements_per_field": s.mesh.num_elements, "steps": steps,
+            "checkpoint_every": checkpoint_every, "forward_s": t_fwd, "backward_s": t_bwd,
+            "recompute_s": st["recompute_ms"] / 1e3, "adjoint_over_forward": t_bwd / t_fwd,
+            "forward_dof_stage_per_s": 3 * dof * steps / t_fwd, "record_forward_s": t_record,
+            "fd_directions": fd_dirs, "fd_max_rel_err": worst, "device_gib": st["device_bytes"] / 2 ** 30}
+
+
 def traffic_from_profile(kernel_prefix: str):
     p = ROOT / "profiles" / "stage_kernel_c2_traffic.json"
     if not p.exists():
@@ -135,6 +192,12 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-cases", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-n", type=int, default=200)
+    ap.add_argument("--adjoint-batch", type=int, default=64)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=20)
+    ap.add_argument("--adjoint-fd", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -237,6 +300,9 @@ def main() -> int:
     }
     if not args.no_batched:
         line["batched"] = batched_bench(args.batched_cases, args.batched_steps)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_batch, args.adjoint_steps,
+                                        args.adjoint_checkpoint, args.adjoint_fd)
     if not args.no_cpu:
         log(f"cpu baseline: oracle at {args.cpu_n}x{args.cpu_n}, {args.cpu_steps} steps")
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)

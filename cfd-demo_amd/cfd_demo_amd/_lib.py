@@ -91,6 +91,7 @@ SIGNATURES = {
     "cfd_adjoint_destroy": (C.c_int, [C.c_void_p]),
     "cfd_adjoint_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cfd_adjoint_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cfd_adjoint_forward_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cfd_adjoint_time_steps": (C.c_int, [C.c_void_p, _dp]),
     "cfd_adjoint_stats": (C.c_int, [C.c_void_p, C.POINTER(CfdAdjointStats)]),
     "cfd_flux_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double, _dp]),
@@ -117,6 +118,13 @@ def load() -> C.CDLL:
     path = Path(os.environ.get("CFDHIP_LIBRARY", LIB_PATH))
     if not path.exists():
         raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (there is no CPU fallback).")
+    try:
+        # torch ships its own libamdhip64; loaded first, it is the one HIP
+        # runtime of the process (a second runtime loaded before it leaves
+        # torch without devices).
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(path))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -148,15 +148,20 @@ class DiscreteAdjoint:
         return out[0] if like.dim() == 3 else out
 
     # -- passes -------------------------------------------------------------------
-    def forward(self, q0):
+    def forward(self, q0, replay: bool = False):
         """qN from q0 (torch CUDA float64, (B, Np, K, 4) or (Np, K, 4));
-        keeps the checkpoints for :meth:`backward`."""
+        keeps the checkpoints and limiter decisions for :meth:`backward`.
+
+        ``replay=True`` instead marches with the troubled-cell decisions of
+        the last plain forward (the frozen map :meth:`backward`
+        differentiates) and keeps nothing: for finite-difference checks."""
         import torch
 
         x = self._to_internal(q0)
         y = torch.empty_like(x)
         torch.cuda.synchronize()
-        _lib.check(_lib.load().cfd_adjoint_forward(self._adj, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
+        fn = _lib.load().cfd_adjoint_forward_replay if replay else _lib.load().cfd_adjoint_forward
+        _lib.check(fn(self._adj, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())))
         return self._to_fields(y, q0)
 
     def backward(self, gn):

@@ -114,16 +114,12 @@ def isentropic_vortex(n: int = 512, *, flux: str = "roe", limiter: str = "unlimi
     return Case(solver, isentropic_vortex_state(x, y, gamma=gamma), final_time)
 
 
-def vortex_bumps(n: int = 200, *, n_bumps: int = 8, seed: int = 0, amplitude: float = 0.05, width: float = 0.5,
-                 flux: str = "rusanov", limiter: str = "unlimited", cfl: float = 0.1, gamma: float = 1.4,
-                 final_time: float = 0.0) -> Case:
-    """Periodic [0,10]^2, n x n split rectangles: the isentropic vortex plus
-    n_bumps seeded density bumps rho += amplitude exp(-|x - x_k|^2 / width^2)
-    (periodic distance; centres uniform in [1, 9]^2 from default_rng(seed)),
-    velocity and pressure of the vortex unchanged; nodal initialisation."""
-    mesh, bcs = periodic_square(n, 10.0)
-    solver = EulerSolver(mesh, bcs, SolverConfig(flux=flux, limiter=limiter, gamma=gamma, cfl=cfl))
-    x, y = solver.node_coordinates()
+def vortex_bumps_state(x, y, *, n_bumps: int = 8, seed: int = 0, amplitude: float = 0.05, width: float = 0.5,
+                       gamma: float = 1.4):
+    """The isentropic vortex plus n_bumps seeded density bumps
+    rho += amplitude exp(-|x - x_k|^2 / width^2) (periodic distance on
+    [0,10]^2; centres uniform in [1, 9]^2 from default_rng(seed)), velocity
+    and pressure of the vortex unchanged: (..., 4)."""
     q = isentropic_vortex_state(x, y, gamma=gamma)
     rho = q[..., 0]
     u, v = q[..., 1] / rho, q[..., 2] / rho
@@ -133,4 +129,16 @@ def vortex_bumps(n: int = 200, *, n_bumps: int = 8, seed: int = 0, amplitude: fl
         dx = (x - cx + 5.0) % 10.0 - 5.0
         dy = (y - cy + 5.0) % 10.0 - 5.0
         rho = rho + amplitude * np.exp(-(dx * dx + dy * dy) / (width * width))
-    return Case(solver, primitive_to_conserved(rho, u, v, p, gamma), final_time)
+    return primitive_to_conserved(rho, u, v, p, gamma)
+
+
+def vortex_bumps(n: int = 200, *, n_bumps: int = 8, seed: int = 0, amplitude: float = 0.05, width: float = 0.5,
+                 flux: str = "rusanov", limiter: str = "unlimited", cfl: float = 0.1, gamma: float = 1.4,
+                 final_time: float = 0.0) -> Case:
+    """Periodic [0,10]^2 on n x n split rectangles with
+    :func:`vortex_bumps_state`, nodal initialisation."""
+    mesh, bcs = periodic_square(n, 10.0)
+    solver = EulerSolver(mesh, bcs, SolverConfig(flux=flux, limiter=limiter, gamma=gamma, cfl=cfl))
+    x, y = solver.node_coordinates()
+    state = vortex_bumps_state(x, y, n_bumps=n_bumps, seed=seed, amplitude=amplitude, width=width, gamma=gamma)
+    return Case(solver, state, final_time)

@@ -124,6 +124,9 @@ struct LimitArgs {
   double* tape = nullptr;
   const int64_t* tape_head = nullptr;
   int64_t tape_cap = 0;
+  // Replay (nullptr: off): the troubled decision of element e is force[e]
+  // instead of the detector's (frozen-decision forward for the adjoint).
+  const uint8_t* force = nullptr;
 };
 
 // Tape record of one limiter worklist element (TAPE_STRIDE doubles):

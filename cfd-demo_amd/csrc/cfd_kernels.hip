@@ -231,6 +231,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     if (a.limit_slopes) {
       double sn[3][4];
       troubled = detect_troubled(a, e, avg, sm, sn);
+      if (a.force) troubled = a.force[e] != 0;
     }
     const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
     const double pbar = pressure_of(avg, a.gamma - 1.0);

@@ -240,6 +240,11 @@ int cfd_adjoint_create(CfdCtx* ctx, int64_t num_steps, const double* dts, int ch
 int cfd_adjoint_destroy(CfdAdjoint* adj);
 int cfd_adjoint_forward(CfdAdjoint* adj, const double* q0_dev, double* qn_dev);
 int cfd_adjoint_backward(CfdAdjoint* adj, const double* gn_dev, double* g0_dev);
+/* The same march with every limiter pass's troubled decisions taken from the
+ * last cfd_adjoint_forward (positivity is applied as usual): the map whose
+ * derivative cfd_adjoint_backward returns, for finite-difference checks.
+ * Leaves the checkpoints untouched. */
+int cfd_adjoint_forward_replay(CfdAdjoint* adj, const double* q0_dev, double* qn_dev);
 int cfd_adjoint_time_steps(CfdAdjoint* adj, double* dts);
 int cfd_adjoint_stats(CfdAdjoint* adj, CfdAdjointStats* stats);
 
