@@ -13,7 +13,7 @@ import numpy as np
 
 from .dg import build_dg_operators, cell_averages, dg_rhs, domain_integral, resolve_boundary_conditions
 from .limiting import apply_limiters, build_limiter_operators
-from .mesh import FixedState, SlipWall, build_connectivity, element_geometry, node_coordinates, \
+from .mesh import FixedState, SlipWall, Transmissive, build_connectivity, element_geometry, node_coordinates, \
     periodic_connectivity, rectangle_mesh
 from .physics import get_interface_flux, get_limiter_function, pressure
 from .reference_element import DEGREE, build_reference_triangle
@@ -225,12 +225,15 @@ SOD_Y_RANGE = (0.0, 0.1)
 SOD_FINAL_TIME = 0.2
 
 
-def sod_shock_tube(nx=200, ny=20, *, flux, limiter, cfl=DEFAULT_CFL):
-    """``cases.py:31-44``: returns (solver, initial_state, final_time)."""
+def sod_shock_tube(nx=200, ny=20, *, flux, limiter, cfl=DEFAULT_CFL, transmissive_ends=False):
+    """``cases.py:31-44``: returns (solver, initial_state, final_time).
+
+    ``transmissive_ends`` (extension) replaces the fixed-state ends."""
     mesh = rectangle_mesh(nx, ny, SOD_X_RANGE, SOD_Y_RANGE)
     left = FixedState.from_primitive(*SOD_LEFT, gamma=SOD_GAMMA)
     right = FixedState.from_primitive(*SOD_RIGHT, gamma=SOD_GAMMA)
-    bcs = {"left": left, "right": right, "bottom": SlipWall(), "top": SlipWall()}
+    ends = (Transmissive(), Transmissive()) if transmissive_ends else (left, right)
+    bcs = {"left": ends[0], "right": ends[1], "bottom": SlipWall(), "top": SlipWall()}
     solver = EulerSolver(mesh, bcs, SolverConfig(flux=flux, limiter=limiter, gamma=SOD_GAMMA, cfl=cfl))
     on_left = solver.geometry.centroid[:, 0] < SOD_DISCONTINUITY
     es = np.where(on_left[:, None], np.array(left.state), np.array(right.state))
