@@ -287,21 +287,24 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
     for (int c = 0; c < 4; ++c) avg[c] = fma(c_avgw[i], q[i][c], avg[c]);
   const double pbar = pressure_of(avg, gm1);
   const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
+  // Nodal pressures and wave speeds with the refined hardware reciprocal and
+  // square root. need_p only sends an element to the repair pass, which
+  // decides with correctly rounded pressures, so it errs towards sending.
+  const double eps_send = eps + fabs(eps) * 1e-12;
   double rmin = INF, pmin = INF, lam = -INF;
   bool finite = true, need_p = false;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
-    double p;
+    const double inv = frcp(q[i][0]);
+    const double u = q[i][1] * inv, v = q[i][2] * inv;
+    const double p = gm1 * (q[i][3] - 0.5 * (q[i][1] * u + q[i][2] * v));
     if constexpr (WITH_LAMBDA) {
-      double ws;
-      pressure_and_speed(q[i], gamma, p, ws);
+      const double ws = fsqrt(u * u + v * v) + fsqrt(gamma * p * inv);
       lam = i == 0 ? ws : nan_max(lam, ws);
-    } else {
-      p = pressure_of(q[i], gm1);
     }
     rmin = nan_min(rmin, q[i][0]);
     pmin = nan_min(pmin, p);
-    need_p = need_p || (p < eps);
+    need_p = need_p || (p < eps_send);
 #pragma unroll
     for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
   }

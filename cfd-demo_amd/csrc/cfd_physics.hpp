@@ -4,8 +4,11 @@
 // Formulas follow src/cfd_demo/euler.py:21-61, fluxes.py:24-143 and
 // limiters.py:16-39. The order of operations follows the reference where it
 // decides branches (HLLC wave-speed signs, Roe entropy fix); elsewhere the
-// compiler may contract a*b+c into FMAs, which moves results by at most a
-// few ulps.
+// compiler may contract a*b+c into FMAs, and the flux divisions and square
+// roots use the hardware reciprocal / reciprocal square root refined by
+// Newton steps (frcp, fdiv, fsqrt below), which moves results by a few ulps.
+// Decisions that compare against thresholds (pressure_of, the limiter) keep
+// correctly rounded arithmetic.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,6 +30,38 @@ enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE 
 constexpr double ROE_ENTROPY_FIX = 0.1;          // fluxes.py:21
 constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.py:24-25
 
+// 1/b from v_rcp_f64 refined by two Newton steps; a/b as a*(1/b) plus one
+// residual correction; sqrt(x) from v_rsq_f64 refined by a coupled
+// Newton step and a residual correction. Each is within a few ulps of the
+// correctly rounded result for the positive, normal arguments the fluxes
+// produce (checked against NumPy in the pointwise-flux tests).
+__device__ __forceinline__ double frcp(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double fdiv(double a, double b) {
+  const double r = frcp(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+__device__ __forceinline__ double fsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * y;
+  double g = x * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  return fma(fma(-g, g, x), h, g);
+}
+// Dual numbers keep exact arithmetic (the adjoint is not on the hot path).
+template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(const ad::Dual<N>& a, const ad::Dual<N>& b) { return a / b; }
+template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(double a, const ad::Dual<N>& b) { return a / b; }
+template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(const ad::Dual<N>& a, double b) { return a / b; }
+template <int N> __device__ __forceinline__ ad::Dual<N> fsqrt(const ad::Dual<N>& a) { return sqrt(a); }
+
 template <class T>
 struct PrimT {
   T rho, u, v, p;
@@ -39,7 +74,7 @@ template <class T>
 __device__ __forceinline__ PrimT<T> to_prim(const T q[4], double gm1) {
   PrimT<T> w;
   w.rho = q[0];
-  w.inv = 1.0 / q[0];
+  w.inv = fdiv(1.0, q[0]);
   w.u = q[1] * w.inv;
   w.v = q[2] * w.inv;
   w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
@@ -86,7 +121,7 @@ __device__ __forceinline__ void make_side(SideT<T>& s, const T q[4], double nx, 
 #pragma unroll
   for (int c = 0; c < 4; ++c) s.q[c] = q[c];
   s.w = to_prim(q, gm1);
-  s.c = sqrt(gamma * s.w.p / s.w.rho);
+  s.c = fsqrt(gamma * s.w.p * s.w.inv);
   s.un = s.w.u * nx + s.w.v * ny;
   normal_flux(q, s.w, nx, ny, s.f);
 }
@@ -112,13 +147,13 @@ __device__ __forceinline__ RoeAvgT<T> roe_average(const SideT<T>& m, const SideT
   RoeAvgT<T> a;
   const T hm = (m.q[3] + m.w.p) * m.w.inv;
   const T hp = (p.q[3] + p.w.p) * p.w.inv;
-  const T wm = sqrt(m.w.rho);
-  const T wp = sqrt(p.w.rho);
-  const T inv = 1.0 / (wm + wp);
+  const T wm = fsqrt(m.w.rho);
+  const T wp = fsqrt(p.w.rho);
+  const T inv = fdiv(1.0, wm + wp);
   a.u = (wm * m.w.u + wp * p.w.u) * inv;
   a.v = (wm * m.w.v + wp * p.w.v) * inv;
   a.h = (wm * hm + wp * hp) * inv;
-  a.c = sqrt(gm1 * (a.h - 0.5 * (a.u * a.u + a.v * a.v)));
+  a.c = fsqrt(gm1 * (a.h - 0.5 * (a.u * a.u + a.v * a.v)));
   a.rho = wm * wp;
   return a;
 }
@@ -135,15 +170,15 @@ __device__ __forceinline__ void flux_roe(const SideT<T>& m, const SideT<T>& p, d
   const T d_un = p.un - m.un;
   const T d_ut = (-p.w.u * ny + p.w.v * nx) - (-m.w.u * ny + m.w.v * nx);
   const T c2 = a.c * a.c;
-  const T inv2c2 = 1.0 / (2.0 * c2);
+  const T inv2c2 = fdiv(1.0, 2.0 * c2);
   const T a_minus = (d_p - a.rho * a.c * d_un) * inv2c2;
-  const T a_entropy = d_rho - d_p / c2;
+  const T a_entropy = d_rho - 2.0 * d_p * inv2c2;
   const T a_shear = a.rho * d_ut;
   const T a_plus = (d_p + a.rho * a.c * d_un) * inv2c2;
   const T delta = ROE_ENTROPY_FIX * (fabs(un) + a.c);
   auto fix = [delta](const T& lam) {
     const T mag = fabs(lam);
-    return mag < delta ? (lam * lam + delta * delta) / (2.0 * delta) : mag;
+    return mag < delta ? fdiv(lam * lam + delta * delta, 2.0 * delta) : mag;
   };
   const T lm = fix(un - a.c) * a_minus;
   const T lp = fix(un + a.c) * a_plus;
@@ -170,7 +205,7 @@ __device__ __forceinline__ void flux_hllc(const SideT<T>& m, const SideT<T>& p, 
   const T s_p = fmax(p.un + p.c, un + a.c);
   const T mass_m = m.w.rho * (s_m - m.un);
   const T mass_p = p.w.rho * (s_p - p.un);
-  const T s_star = (p.w.p - m.w.p + mass_m * m.un - mass_p * p.un) / safe_den(mass_m - mass_p);
+  const T s_star = fdiv(p.w.p - m.w.p + mass_m * m.un - mass_p * p.un, safe_den(mass_m - mass_p));
   if (s_m >= 0.0) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = m.f[c];
@@ -185,13 +220,13 @@ __device__ __forceinline__ void flux_hllc(const SideT<T>& m, const SideT<T>& p, 
   const SideT<T>& k = left ? m : p;
   const T s_k = left ? s_m : s_p;
   const T mass_k = left ? mass_m : mass_p;
-  const T factor = mass_k / safe_den(s_k - s_star);
+  const T factor = fdiv(mass_k, safe_den(s_k - s_star));
   const T shift = s_star - k.un;
   T star[4];
   star[0] = factor;
   star[1] = factor * (k.w.u + shift * nx);
   star[2] = factor * (k.w.v + shift * ny);
-  star[3] = factor * (k.q[3] * k.w.inv + shift * (s_star + k.w.p / safe_den(mass_k)));
+  star[3] = factor * (k.q[3] * k.w.inv + shift * (s_star + fdiv(k.w.p, safe_den(mass_k))));
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[c] = k.f[c] + s_k * (star[c] - k.q[c]);
 }
