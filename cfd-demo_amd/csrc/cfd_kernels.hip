@@ -11,13 +11,17 @@
 //   k_finalize     deterministic two-level reduction of per-block partials
 //
 // State layout in HBM is the reference's (Np, K, 4) float64, node-major:
-// node j of element e starts at q + (j*K + e)*4. A stage workgroup owns SB
-// consecutive elements, one lane each, so every per-node load of a wave is
-// one contiguous 2 KiB run. The 15x15 operators live in __constant__ memory
-// and are read with wave-uniform indices, i.e. as scalar FMA operands; the 60
-// accumulators stay in registers (two waves per SIMD). Staging the block's
-// state in LDS (one wave per SIMD) and a 16-lanes-per-element FP64 MFMA tile
-// variant were both measured slower on MI355X and are not kept.
+// node j of element e starts at q + (j*K + e)*4. The 15x15 operators live in
+// __constant__ memory and are read with wave-uniform indices, i.e. as scalar
+// FMA operands.
+//
+// Two stage-kernel layouts: k_stage4 (cfd_stage4.inc; four lanes per element,
+// 15 accumulators per lane, three waves per SIMD) is the default for a fixed
+// flux; k_stage below (one lane per element, 60 accumulators, two waves per
+// SIMD) serves the run-time flux of batched solves, the summary-only mode and
+// other workgroup sizes. A 16-lanes-per-element FP64 MFMA tile variant and an
+// LDS-staged one-lane-per-element variant were both measured slower on MI355X
+// and are not kept.
 #include "cfd_device.hpp"
 
 namespace cfd {
@@ -469,6 +473,7 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 
 }  // namespace cfd
 
+#include "cfd_stage4.inc"
 #include "cfd_adjoint_kernels.inc"
 #include "cfd_capi.inc"
 #include "cfd_batched.inc"
