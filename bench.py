@@ -258,7 +258,8 @@ def main() -> int:
     log(f"stage kernel {stage_ms:.3f} ms ({stage_gbs:.0f} GB/s), limiter {limit_ms:.3f} ms, "
         f"rhs kernel {rhs_ms.value:.3f} ms ({rhs_gbs:.0f} GB/s)")
 
-    traffic = traffic_from_profile("k_stage")
+    traffic = traffic_from_profile("stage_kernel")
+    rhs_traffic = traffic_from_profile("rhs_kernel")
     line = {
         "metric": METRIC,
         "value": value,
@@ -286,13 +287,13 @@ def main() -> int:
             "unit": "GB/s",
             "frac": stage_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_stage<HLLC> (RHS + SSP-RK3 combine + limiter summary), mean of the 3 stage launches",
+            "kernel": "k_stage4<HLLC> (RHS + SSP-RK3 combine + limiter summary, four lanes per element), mean of the 3 stage launches",
             "algorithmic_bytes_per_launch": STAGE_BYTES_PER_DOF * dof,
             "mean_launch_ms": stage_ms,
         },
         "rhs_roofline": {
             "bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS,
-            "kernel": "k_stage<HLLC, RHS only>", "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
+            "kernel": "k_stage4<HLLC, RHS only>", "traffic": rhs_traffic, "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
             "mean_launch_ms": rhs_ms.value,
         },
         "limiter_mean_launch_ms": limit_ms,
