@@ -143,9 +143,11 @@ class BatchedSolver:
         _lib.check(_bind().cfd_batch_begin(self._h, _lib.dptr(self._q0)))
 
     def autotune(self, reps: int = 3):
-        """Pick elements per stage workgroup by timing; returns (choice, ms for 64/128/256)."""
+        """Pick the stage-kernel layout by timing; returns (code, ms for codes
+        64/128/256 (one lane per element, elements per workgroup) and 4 (four
+        lanes per element))."""
         ch = C.c_int(0)
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         _lib.check(_bind().cfd_batch_autotune(self._h, reps, C.byref(ch), _lib.dptr(ms)))
         return ch.value, ms
 

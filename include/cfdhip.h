@@ -199,8 +199,10 @@ int cfd_batch_begin(CfdBatch* batch, const double* q);
 int cfd_batch_advance(CfdBatch* batch, int64_t nsteps, int use_graph, int64_t* active_left);
 int cfd_batch_download(CfdBatch* batch, double* q);
 int cfd_batch_results(CfdBatch* batch, CfdBatchCaseResult* results);
-/* Autotuner: time the stage kernel at 64/128/256 elements per workgroup on
- * the current state and keep the fastest (ms_out[3] may be NULL). */
+/* Autotuner: time the stage kernel on the current state in its four layouts
+ * and keep the fastest: one lane per element with 64, 128 or 256 elements
+ * per workgroup (codes 64, 128, 256) or four lanes per element (code 4).
+ * chosen receives the code; ms_out[4] (may be NULL) the mean times. */
 int cfd_batch_autotune(CfdBatch* batch, int reps, int* chosen, double* ms_out);
 
 /* Discrete adjoint (no reference counterpart: the reference differentiates

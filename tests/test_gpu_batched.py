@@ -21,11 +21,16 @@ def _cases(n, seed, final_time=None, nx_range=(50, 90)):
     return [shock_tube_case(s) for s in specs]
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batched_matches_single_case_runs(use_graph):
+@pytest.mark.parametrize("use_graph,quad", [(True, True), (False, True), (True, False)])
+def test_batched_matches_single_case_runs(use_graph, quad, monkeypatch):
+    """Both stage-kernel layouts (four lanes per element, the default, and one
+    lane per element) against the single-case runs."""
     cases = _cases(6, seed=3, final_time=0.004)
-    print(f"case batched use_graph={use_graph} K={[c.solver.mesh.num_elements for c in cases]}", flush=True)
+    print(f"case batched use_graph={use_graph} quad={quad} K={[c.solver.mesh.num_elements for c in cases]}",
+          flush=True)
+    monkeypatch.setenv("CFDHIP_STAGE_QUAD", "1" if quad else "0")
     batch = BatchedSolver(cases)
+    monkeypatch.delenv("CFDHIP_STAGE_QUAD")
     got = batch.run(use_graph=use_graph)
     for case, r in zip(cases, got):
         ref = case.solver.run(case.initial_state, case.final_time)
@@ -46,7 +51,7 @@ def test_autotuner_and_rejection_case():
     batch = BatchedSolver(cases + [bad])
     batch.begin()
     choice, ms = batch.autotune(reps=2)
-    assert choice in (64, 128, 256) and np.all(ms > 0)
+    assert choice in (64, 128, 256, 4) and np.all(ms > 0)
     while batch.advance(64) > 0:
         pass
     res = batch.results()
