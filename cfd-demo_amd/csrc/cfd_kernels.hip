@@ -474,6 +474,7 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 }  // namespace cfd
 
 #include "cfd_stage4.inc"
+#include "cfd_stage_mfma.inc"
 #include "cfd_adjoint_kernels.inc"
 #include "cfd_capi.inc"
 #include "cfd_batched.inc"

@@ -18,6 +18,16 @@ FLUXES = ["rusanov", "roe", "hllc"]
 BCS = {"sod": sod_bcs, "fixed": fixed_all, "trans": trans_all, "mixed": mixed_bcs, "periodic": periodic_bcs}
 
 
+@pytest.fixture(autouse=True, params=["quad", "mfma", "lane"])
+def stage_layout(request, monkeypatch):
+    """Every parity test under each stage-kernel layout: four lanes per
+    element (the default), the MFMA contractions, one lane per element."""
+    monkeypatch.setenv("CFDHIP_STAGE_QUAD", "0" if request.param == "lane" else "1")
+    monkeypatch.setenv("CFDHIP_STAGE_MFMA", "1" if request.param == "mfma" else "0")
+    print(f"layout {request.param}", flush=True)
+    return request.param
+
+
 def _pair(bcname, flux="rusanov", limiter="minmod", nx=6, ny=5, seed=3, cfl=0.5):
     v, e, b = perturbed_vertices(nx, ny, seed=seed)
     if bcname == "periodic":
