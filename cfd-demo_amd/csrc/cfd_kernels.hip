@@ -15,13 +15,15 @@
 // __constant__ memory and are read with wave-uniform indices, i.e. as scalar
 // FMA operands.
 //
-// Two stage-kernel layouts: k_stage4 (cfd_stage4.inc; four lanes per element,
-// 15 accumulators per lane, three waves per SIMD) is the default for a fixed
-// flux; k_stage below (one lane per element, 60 accumulators, two waves per
-// SIMD) serves the run-time flux of batched solves, the summary-only mode and
-// other workgroup sizes. A 16-lanes-per-element FP64 MFMA tile variant and an
-// LDS-staged one-lane-per-element variant were both measured slower on MI355X
-// and are not kept.
+// Three stage-kernel layouts: k_stage4 (cfd_stage4.inc; four lanes per
+// element, 15 accumulators per lane, three waves per SIMD) is the default for
+// a fixed flux; k_stage below (one lane per element, 60 accumulators, two
+// waves per SIMD) serves the run-time flux of batched solves, the
+// summary-only mode and other workgroup sizes; k_stagem (cfd_stage_mfma.inc)
+// runs the contractions on FP64 MFMA and is opt-in (slower than k_stage4 on
+// MI355X, whose FP64 matrix and vector rates are equal). A
+// 16-lanes-per-element MFMA tile variant and an LDS-staged
+// one-lane-per-element variant were also measured slower and are not kept.
 #include "cfd_device.hpp"
 
 namespace cfd {
