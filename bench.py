@@ -180,6 +180,60 @@ def traffic_from_profile(kernel_prefix: str):
         return None
 
 
+def cases_bench(c3_n: int) -> dict:
+    """Full runs through ``EulerSolver.run``: the reference's Sod case (c1:
+    100x10, Rusanov, minmod, CFL 0.1, t = 0.2) and the isentropic vortex (c3:
+    periodic [0,10]^2, c3_n^2 rectangles, Roe, unlimited, CFL 0.5, t = 2),
+    each with its L1 density error against the exact solution and its
+    conservation defect."""
+    from cfd_demo_amd.cases import isentropic_vortex, isentropic_vortex_state, sod_shock_tube
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    from sod_exact import sample
+
+    def l1(solver, rho, rho_ex, area):
+        w = solver.reference.cell_weights[:, None] * solver.geometry.jacobian[None, :]
+        return float(np.sum(w * np.abs(rho - rho_ex)) / area)
+
+    out = {}
+    case = sod_shock_tube(100, 10, flux="rusanov", limiter="minmod", cfl=0.1)
+    s, q0, T = case.solver, case.initial_state, case.final_time
+    s.run(q0, 1e-3)  # context and tables on the device
+    t0 = time.perf_counter()
+    r = s.run(q0, T)
+    wall = time.perf_counter() - t0
+    x, _ = s.node_coordinates()
+    xu, inv = np.unique(x, return_inverse=True)
+    rho_ex = sample(xu, T, 0.5, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1))[0][inv].reshape(x.shape)
+    defect = s.integrate(r.state) - s.integrate(q0) + r.boundary_flux_integral
+    out["c1_sod"] = {"elements": s.mesh.num_elements, "steps": r.steps, "wall_s": wall,
+                     "ms_per_step": 1e3 * wall / r.steps, "l1_density_vs_exact": l1(s, r.state[..., 0], rho_ex, 0.1),
+                     "conservation_defect": float(np.abs(defect).max())}
+    log(f"c1 Sod 100x10: {r.steps} steps in {wall:.2f}s ({1e3 * wall / r.steps:.3f} ms/step), "
+        f"L1 {out['c1_sod']['l1_density_vs_exact']:.3e}, defect {out['c1_sod']['conservation_defect']:.1e}")
+    s.close()
+
+    case = isentropic_vortex(c3_n, flux="roe", limiter="unlimited")
+    s, q0, T = case.solver, case.initial_state, case.final_time
+    s.run(q0, 1e-3)
+    t0 = time.perf_counter()
+    r = s.run(q0, T)
+    wall = time.perf_counter() - t0
+    x, y = s.node_coordinates()
+    exact = isentropic_vortex_state(x, y, T)
+    defect = s.integrate(r.state) - s.integrate(q0) + r.boundary_flux_integral
+    K = s.mesh.num_elements
+    out["c3_vortex"] = {"elements": K, "steps": r.steps, "wall_s": wall, "ms_per_step": 1e3 * wall / r.steps,
+                        "dof_stage_per_s": 3.0 * 60 * K * r.steps / wall,
+                        "l1_density_vs_exact": l1(s, r.state[..., 0], exact[..., 0], 100.0),
+                        "conservation_defect": float(np.abs(defect).max())}
+    log(f"c3 vortex {c3_n}^2 ({K} elements): {r.steps} steps in {wall:.2f}s "
+        f"({1e3 * wall / r.steps:.3f} ms/step, {out['c3_vortex']['dof_stage_per_s']:.3e} DOF-stage/s), "
+        f"L1 {out['c3_vortex']['l1_density_vs_exact']:.3e}, defect {out['c3_vortex']['conservation_defect']:.1e}")
+    s.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +247,8 @@ def main() -> int:
     ap.add_argument("--batched-cases", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-cases", action="store_true", help="skip the full c1 (Sod) and c3 (vortex) runs")
+    ap.add_argument("--c3-n", type=int, default=512)
     ap.add_argument("--adjoint-n", type=int, default=200)
     ap.add_argument("--adjoint-batch", type=int, default=64)
     ap.add_argument("--adjoint-steps", type=int, default=200)
@@ -304,6 +360,8 @@ def main() -> int:
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_batch, args.adjoint_steps,
                                         args.adjoint_checkpoint, args.adjoint_fd)
+    if not args.no_cases:
+        line["cases"] = cases_bench(args.c3_n)
     if not args.no_cpu:
         log(f"cpu baseline: oracle at {args.cpu_n}x{args.cpu_n}, {args.cpu_steps} steps")
         line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
