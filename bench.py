@@ -314,6 +314,33 @@ def main() -> int:
     log(f"stage kernel {stage_ms:.3f} ms ({stage_gbs:.0f} GB/s), limiter {limit_ms:.3f} ms, "
         f"rhs kernel {rhs_ms.value:.3f} ms ({rhs_gbs:.0f} GB/s)")
 
+    # RHS on two inputs, median of 20 single-launch timings after 3 warm-ups:
+    # the evolved state, and the same state with a seeded +-1 % multiplicative
+    # perturbation of density and pressure (no flat regions with trivial fluxes).
+    def rhs_median(reps: int = 20) -> float:
+        one = C.c_double(0.0)
+        for _ in range(3):
+            _lib.check(lib.cfd_time_rhs(ctx, 1, C.byref(one)))
+        samples = []
+        for _ in range(reps):
+            _lib.check(lib.cfd_time_rhs(ctx, 1, C.byref(one)))
+            samples.append(one.value)
+        return float(np.median(samples))
+
+    rhs_med = rhs_median()
+    q_ev = np.empty_like(q0)
+    _lib.check(lib.cfd_state_download(ctx, _lib.dptr(q_ev)))
+    rng = np.random.default_rng(0)
+    gam = solver.config.gamma
+    rho = q_ev[..., 0] * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, q_ev.shape[:2]))
+    u, v = q_ev[..., 1] / q_ev[..., 0], q_ev[..., 2] / q_ev[..., 0]
+    p = (gam - 1.0) * (q_ev[..., 3] - 0.5 * (q_ev[..., 1] * u + q_ev[..., 2] * v))
+    p = p * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, q_ev.shape[:2]))
+    q_pt = np.ascontiguousarray(np.stack([rho, rho * u, rho * v, p / (gam - 1.0) + 0.5 * rho * (u * u + v * v)], -1))
+    _lib.check(lib.cfd_state_upload(ctx, _lib.dptr(q_pt)))
+    rhs_med_pt = rhs_median()
+    log(f"rhs kernel median of 20: {rhs_med:.3f} ms (evolved state), {rhs_med_pt:.3f} ms (perturbed)")
+
     traffic = traffic_from_profile("stage_kernel")
     rhs_traffic = traffic_from_profile("rhs_kernel")
     line = {
@@ -351,6 +378,8 @@ def main() -> int:
             "bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS,
             "kernel": "k_stage4<HLLC, RHS only>", "traffic": rhs_traffic, "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
             "mean_launch_ms": rhs_ms.value,
+            "median_launch_ms": rhs_med,
+            "median_launch_ms_perturbed_input": rhs_med_pt,
         },
         "limiter_mean_launch_ms": limit_ms,
         "cpu_baseline": None,
