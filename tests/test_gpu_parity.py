@@ -159,3 +159,29 @@ def test_pointwise_flux_matches_oracle(flux):
     terms = (np.abs(normal_flux(qm, nx, ny, 1.4)) + np.abs(normal_flux(qp, nx, ny, 1.4))
              + np.maximum(speed(qm), speed(qp))[:, None] * np.abs(qp - qm))
     assert np.max(np.abs(got - want) / terms.max(axis=1, keepdims=True)) <= 1e-13
+
+
+def test_run_error_paths_match_oracle():
+    """The reference's run-loop failures (solver.py:171-197) through the C ABI,
+    with the oracle's exception class and message: an inadmissible initial
+    state, and a step rejected max_rejections + 1 times (CFL 1e6)."""
+    import cfd_demo_amd as gpu
+    from oracle import solver as osol
+
+    v, e, b = perturbed_vertices(4, 2, seed=3)
+    g, o = make_pair(v, e, b, sod_bcs(), cfl=1e6, max_rejections=2)
+    left = g.geometry.centroid[:, 0] < 0.5
+    q = np.empty(g.state_shape)
+    q[:, left] = [1.0, 0.0, 0.0, 2.5]
+    q[:, ~left] = [0.125, 0.0, 0.0, 0.25]
+    bad = q.copy()
+    bad[0, 0, 0] = -1.0
+    print("case run error paths", flush=True)
+    msgs = []
+    for s, err in ((g, gpu.SolverError), (o, osol.SolverError)):
+        with pytest.raises(err, match="positive nodal density and pressure") as e1:
+            s.run(bad, 0.01)
+        with pytest.raises(err, match="was rejected 3 times") as e2:
+            s.run(q, 0.5)
+        msgs.append((str(e1.value), str(e2.value)))
+    assert msgs[0] == msgs[1]
