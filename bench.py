@@ -68,7 +68,14 @@ def cpu_baseline(n: int, steps: int) -> dict:
         q, rep = s.step(q, s.time_step(q))
     el = time.perf_counter() - t0
     dof = 60 * s.mesh.num_elements
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {"value": 3 * dof * steps / el, "unit": "DOF-stage-updates/s", "cores": 1, "kind": "port",
+            "cpu_model": model,
             "sample": f"oracle (NumPy restatement of the reference JAX path) {steps} SSP-RK3 steps incl. CFL dt "
                       f"on the same four-quadrant HLLC/MC workload at {n}x{n} rectangles ({s.mesh.num_elements} "
                       f"triangles), {el:.1f} s"}
