@@ -185,3 +185,13 @@ def test_run_error_paths_match_oracle():
             s.run(q, 0.5)
         msgs.append((str(e1.value), str(e2.value)))
     assert msgs[0] == msgs[1]
+
+
+def test_averages_and_domain_integral_match_oracle():
+    """cell_averages and integrate (dg.py:168-176) through the C ABI."""
+    g, o = _pair("mixed", nx=7, ny=4)
+    q = random_state(g.state_shape, 5)
+    print("case averages and domain integral", flush=True)
+    assert np.allclose(g.cell_averages(q), o.cell_averages(q), rtol=1e-14, atol=1e-15)
+    ig, io = g.integrate(q), o.integrate(q)
+    assert np.allclose(ig, io, rtol=1e-14, atol=0.0)
