@@ -73,10 +73,14 @@ class BatchedSolver:
             raise ValueError("at least one case is needed")
         self.cases = list(cases)
         self.gamma = gamma
+        # Cases are laid out grouped by flux (so each flux's stage launch spans
+        # only its own blocks); results come back in the caller's order.
+        self._order = sorted(range(len(self.cases)), key=lambda i: self.cases[i].solver.flux.id)
         geo, conn, lim, states, q_parts, specs = [], [], [], [], [], []
         first = 0
         nstates = 0
-        for i, case in enumerate(self.cases):
+        for i in self._order:
+            case = self.cases[i]
             s = case.solver
             if s.config.gamma != gamma:
                 raise ValueError("all cases of a batch share gamma")
@@ -112,8 +116,10 @@ class BatchedSolver:
                                       int(s.config.max_rejections)))
             first += npad
         self.K = first
-        self.first = [sp.first for sp in specs]
-        self.count = [sp.count for sp in specs]
+        self.first = [0] * len(specs)
+        self.count = [0] * len(specs)
+        for pos, i in enumerate(self._order):
+            self.first[i], self.count[i] = specs[pos].first, specs[pos].count
         self._geo = np.ascontiguousarray(np.concatenate(geo))
         self._conn = np.ascontiguousarray(np.concatenate(conn))
         self._lim = np.ascontiguousarray(np.concatenate(lim))
@@ -144,10 +150,11 @@ class BatchedSolver:
 
     def autotune(self, reps: int = 3):
         """Pick the stage-kernel layout by timing; returns (code, ms for codes
-        64/128/256 (one lane per element, elements per workgroup) and 4 (four
-        lanes per element))."""
+        64/128/256 (one lane per element, elements per workgroup), 4 (four
+        lanes per element, run-time flux) and 5 (four lanes per element, one
+        launch per flux))."""
         ch = C.c_int(0)
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         _lib.check(_bind().cfd_batch_autotune(self._h, reps, C.byref(ch), _lib.dptr(ms)))
         return ch.value, ms
 
@@ -159,7 +166,10 @@ class BatchedSolver:
     def results(self):
         res = (CfdBatchCaseResult * len(self.cases))()
         _lib.check(_bind().cfd_batch_results(self._h, res))
-        return list(res)
+        out = [None] * len(self.cases)
+        for pos, i in enumerate(self._order):
+            out[i] = res[pos]
+        return out
 
     def states(self):
         q = np.empty_like(self._q0)

@@ -98,7 +98,8 @@ struct StageArgs {
   int64_t K;
   double gamma;
   double dt;
-  BatchView bv;  // used only when FLUX == FLUX_DYNAMIC
+  BatchView bv;  // batched solves only
+  int blk0 = 0;  // batched one-flux launches: first element block of the launch
 };
 
 struct LimitArgs {

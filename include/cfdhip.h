@@ -199,10 +199,12 @@ int cfd_batch_begin(CfdBatch* batch, const double* q);
 int cfd_batch_advance(CfdBatch* batch, int64_t nsteps, int use_graph, int64_t* active_left);
 int cfd_batch_download(CfdBatch* batch, double* q);
 int cfd_batch_results(CfdBatch* batch, CfdBatchCaseResult* results);
-/* Autotuner: time the stage kernel on the current state in its four layouts
+/* Autotuner: time the stage kernel on the current state in its five layouts
  * and keep the fastest: one lane per element with 64, 128 or 256 elements
- * per workgroup (codes 64, 128, 256) or four lanes per element (code 4).
- * chosen receives the code; ms_out[4] (may be NULL) the mean times. */
+ * per workgroup (codes 64, 128, 256), four lanes per element with the flux
+ * chosen at run time (code 4) or one four-lane launch per flux present
+ * (code 5). chosen receives the code; ms_out[5] (may be NULL) the mean
+ * times. */
 int cfd_batch_autotune(CfdBatch* batch, int reps, int* chosen, double* ms_out);
 
 /* Discrete adjoint (no reference counterpart: the reference differentiates

@@ -21,16 +21,20 @@ def _cases(n, seed, final_time=None, nx_range=(50, 90)):
     return [shock_tube_case(s) for s in specs]
 
 
-@pytest.mark.parametrize("use_graph,quad", [(True, True), (False, True), (True, False)])
-def test_batched_matches_single_case_runs(use_graph, quad, monkeypatch):
-    """Both stage-kernel layouts (four lanes per element, the default, and one
-    lane per element) against the single-case runs."""
+@pytest.mark.parametrize("use_graph,quad,per_flux", [(True, True, False), (False, True, False), (True, False, False),
+                                                     (True, True, True), (False, True, True)])
+def test_batched_matches_single_case_runs(use_graph, quad, per_flux, monkeypatch):
+    """The stage-kernel layouts (four lanes per element with the run-time
+    flux, the default, or one launch per flux; one lane per element) against
+    the single-case runs."""
     cases = _cases(6, seed=3, final_time=0.004)
-    print(f"case batched use_graph={use_graph} quad={quad} K={[c.solver.mesh.num_elements for c in cases]}",
-          flush=True)
+    print(f"case batched use_graph={use_graph} quad={quad} per_flux={per_flux} "
+          f"K={[c.solver.mesh.num_elements for c in cases]}", flush=True)
     monkeypatch.setenv("CFDHIP_STAGE_QUAD", "1" if quad else "0")
+    monkeypatch.setenv("CFDHIP_STAGE_PER_FLUX", "1" if per_flux else "0")
     batch = BatchedSolver(cases)
     monkeypatch.delenv("CFDHIP_STAGE_QUAD")
+    monkeypatch.delenv("CFDHIP_STAGE_PER_FLUX")
     got = batch.run(use_graph=use_graph)
     for case, r in zip(cases, got):
         ref = case.solver.run(case.initial_state, case.final_time)
@@ -51,7 +55,7 @@ def test_autotuner_and_rejection_case():
     batch = BatchedSolver(cases + [bad])
     batch.begin()
     choice, ms = batch.autotune(reps=2)
-    assert choice in (64, 128, 256, 4) and np.all(ms > 0)
+    assert choice in (64, 128, 256, 4, 5) and np.all(ms > 0)
     while batch.advance(64) > 0:
         pass
     res = batch.results()
