@@ -228,8 +228,23 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     a.limiter = a.bv.limiter[cs];
     a.limit_slopes = a.limiter != LIM_UNLIMITED;
   }
+  // The block's summary rows through LDS: contiguous 16-byte loads across the
+  // block instead of one 128-byte row per lane (rows padded to 18 doubles so
+  // the per-row reads below are free of bank conflicts).
+  __shared__ double srow[LBLOCK][18];
+  {
+    const int64_t r0 = (int64_t)blk * LBLOCK;
+    const int nrow = (int)min((int64_t)LBLOCK, a.K - r0);
+    const double2* src = reinterpret_cast<const double2*>(a.summ + r0 * SUMM_STRIDE);
+    for (int i = threadIdx.x; i < nrow * (SUMM_STRIDE / 2); i += LBLOCK) {
+      const double2 v = src[i];
+      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2))] = v.x;
+      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2)) + 1] = v.y;
+    }
+    __syncthreads();
+  }
   if (live) {
-    const double* sm = a.summ + e * SUMM_STRIDE;
+    const double* sm = &srow[threadIdx.x][0];
     double avg[4];
     load4(sm, avg);
     const int fl = (int)sm[SUMM_FLAGS];
