@@ -397,100 +397,4 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
   return troubled;
 }
 
-// Cockburn-Shu P1 reconstruction of a troubled cell (limiting.py:153-163).
-__device__ __forceinline__ void reconstruct_p1(double q[NP][4], const double avg[4], const double sn[3][4],
-                                               int limiter) {
-#pragma clang fp contract(off)
-  double d[3][4];
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double s = 0.0;
-#pragma unroll
-      for (int i = 0; i < NP; ++i) s = s + c_p1mid[f * NP + i] * q[i][c];
-      const double proj = s - avg[c];
-      if (proj != 0.0) {
-        const double ratio = sn[f][c] / proj;
-        d[f][c] = proj * fmin(1.0, limiter_phi(limiter, ratio));
-      } else {
-        d[f][c] = 0.0;
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double pos = (fmax(d[0][c], 0.0) + fmax(d[1][c], 0.0)) + fmax(d[2][c], 0.0);
-    const double neg = (fmax(-d[0][c], 0.0) + fmax(-d[1][c], 0.0)) + fmax(-d[2][c], 0.0);
-    const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
-    const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) d[f][c] = tp * fmax(d[f][c], 0.0) - tn * fmax(-d[f][c], 0.0);
-  }
-#pragma unroll
-  for (int i = 0; i < NP; ++i)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const double lin = (c_linb[i * 3 + 0] * d[0][c] + c_linb[i * 3 + 1] * d[1][c]) + c_linb[i * 3 + 2] * d[2][c];
-      q[i][c] = avg[c] + lin;
-    }
-}
-
-// Zhang-Shu positivity scaling (limiting.py:166-200), per element. Returns the
-// overall scale theta of q - avg that was applied (1 when nothing changed).
-__device__ __forceinline__ double positivity(double q[NP][4], const double avg[4], double gamma) {
-#pragma clang fp contract(off)
-  const double gm1 = gamma - 1.0;
-  const double pbar = pressure_of(avg, gm1);
-  const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
-  double rmin = INF;
-  double scale = 1.0;
-#pragma unroll
-  for (int i = 0; i < NP; ++i) rmin = nan_min(rmin, q[i][0]);
-  if (rmin < eps) {
-    const double span = avg[0] - rmin;
-    double theta = (avg[0] - eps) / span;
-    theta = fmin(fmax(theta, 0.0), 1.0);
-    scale = theta;
-#pragma unroll
-    for (int i = 0; i < NP; ++i)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) q[i][c] = avg[c] + theta * (q[i][c] - avg[c]);
-  }
-  bool any = false;
-  bool need[NP];
-#pragma unroll
-  for (int i = 0; i < NP; ++i) {
-    need[i] = pressure_of(q[i], gm1) < eps;
-    any = any || need[i];
-  }
-  if (!any) return scale;
-  double theta = 1.0;
-  for (int i = 0; i < NP; ++i) {
-    if (!need[i]) continue;
-    double dev[4];
-    for (int c = 0; c < 4; ++c) dev[c] = q[i][c] - avg[c];
-    double lo = 0.0, hi = 1.0;
-    for (int it = 0; it < 60; ++it) {
-      const double mid = 0.5 * (lo + hi);
-      double s[4];
-      for (int c = 0; c < 4; ++c) s[c] = avg[c] + mid * dev[c];
-      if (pressure_of(s, gm1) >= eps) {
-        lo = mid;
-      } else {
-        hi = mid;
-      }
-    }
-    theta = fmin(theta, lo);
-  }
-  if (theta < 1.0) {
-#pragma unroll
-    for (int i = 0; i < NP; ++i)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) q[i][c] = avg[c] + theta * (q[i][c] - avg[c]);
-    scale = scale * theta;
-  }
-  return scale;
-}
-
 }  // namespace cfd

@@ -283,20 +283,55 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
 // Limiter, pass 2: reconstruction and positivity for worklist elements. A
 // fixed grid strides over the worklist; every lane reaches the exit.
 // ---------------------------------------------------------------------------
-// Repair one worklist element in place; fold its report values into row.
+// Repair of worklist elements by groups of RG = 16 lanes: lane i of a group
+// owns node i (lane 15 repeats node 14), so an element's repair takes the
+// latency of one node's arithmetic instead of fifteen. Element-wide values
+// (the detection, the P1 projections at the edge midpoints, the Cockburn-Shu
+// and positivity scales) are evaluated with the operations and order of a
+// one-lane loop over the nodes (projections summed in node order from the
+// group's node buffer), so the repaired state does not depend on the grouping;
+// minima, maxima and "all finite" over the nodes go through 16-lane shuffles.
 // With a tape, worklist slot `slot` also records the pre-limit values, the
 // neighbour deviations and the positivity scale for the adjoint.
-__device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, int64_t slot, double row[PART_COLS]) {
+constexpr int RG = 16;  // lanes per repaired element
+
+// Group-local LDS exchange: a group lies inside one wave.
+__device__ __forceinline__ void grp_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ double grp_min(double v) {
+#pragma unroll
+  for (int off = RG / 2; off >= 1; off >>= 1) v = nan_min(v, __shfl_xor(v, off, RG));
+  return v;
+}
+__device__ __forceinline__ double grp_max(double v) {
+#pragma unroll
+  for (int off = RG / 2; off >= 1; off >>= 1) v = nan_max(v, __shfl_xor(v, off, RG));
+  return v;
+}
+__device__ __forceinline__ bool grp_any(bool b) {
+  int v = b ? 1 : 0;
+#pragma unroll
+  for (int off = RG / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off, RG);
+  return v != 0;
+}
+
+// qs: the group's node buffer [NP][4]; ds: its 12 midpoint slopes.
+__device__ __forceinline__ void repair_group(const LimitArgs& a, int entry, int64_t slot, int lane, double (*qs)[4],
+                                             double* ds, double row[PART_COLS]) {
+#pragma clang fp contract(off)
   const double gm1 = a.gamma - 1.0;
   const int64_t e = entry & 0x7fffffff;
   const bool troubled = entry < 0;
   const int64_t K = a.K;
+  const int j = min(lane, NP - 1);
   const double* sm = a.summ + e * SUMM_STRIDE;
   double avg[4];
   load4(sm, avg);
-  double q[NP][4];
-#pragma unroll
-  for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, q[i]);
+  double q[4];
+  load4(a.q + ((int64_t)j * K + e) * 4, q);
   double sn[3][4];
   if (troubled) detect_troubled(a, e, avg, sm, sn);
   double* rec = nullptr;
@@ -305,47 +340,131 @@ __device__ __forceinline__ void repair_element(const LimitArgs& a, int entry, in
     if (pos < a.tape_cap) rec = a.tape + pos * TAPE_STRIDE;
   }
   if (rec) {
-    rec[TAPE_E] = (double)e;
-    rec[TAPE_TROUBLED] = troubled ? 1.0 : 0.0;
+    if (lane == 0) {
+      rec[TAPE_E] = (double)e;
+      rec[TAPE_TROUBLED] = troubled ? 1.0 : 0.0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[TAPE_SN + 4 * f + c] = troubled ? sn[f][c] : 0.0;
+    }
+    if (lane < NP) store4(rec + TAPE_Q + 4 * lane, q);
+  }
+  if (troubled) {
+    // Cockburn-Shu P1 reconstruction (limiting.py:153-163): lane 4f + c
+    // projects component c at midpoint f, every lane rebuilds its node.
+    store4(qs[j], q);
+    grp_sync();
+    if (lane < 12) {
+      const int f = lane >> 2, c = lane & 3;
+      double s = 0.0;
+#pragma unroll
+      for (int n = 0; n < NP; ++n) s = s + c_p1mid[f * NP + n] * qs[n][c];
+      double snv = 0.0;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) snv = k == lane ? sn[k >> 2][k & 3] : snv;
+      const double proj = s - sm[c];
+      double dv = 0.0;
+      if (proj != 0.0) {
+        const double ratio = snv / proj;
+        dv = proj * fmin(1.0, limiter_phi(a.limiter, ratio));
+      }
+      ds[lane] = dv;
+    }
+    grp_sync();
+    double d[3][4];
 #pragma unroll
     for (int f = 0; f < 3; ++f)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) rec[TAPE_SN + 4 * f + c] = troubled ? sn[f][c] : 0.0;
+      for (int c = 0; c < 4; ++c) d[f][c] = ds[4 * f + c];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) store4(rec + TAPE_Q + 4 * i, q[i]);
+    for (int c = 0; c < 4; ++c) {
+      const double pos = (fmax(d[0][c], 0.0) + fmax(d[1][c], 0.0)) + fmax(d[2][c], 0.0);
+      const double neg = (fmax(-d[0][c], 0.0) + fmax(-d[1][c], 0.0)) + fmax(-d[2][c], 0.0);
+      const double tp = pos > 0.0 ? fmin(1.0, neg / pos) : 0.0;
+      const double tn = neg > 0.0 ? fmin(1.0, pos / neg) : 0.0;
+#pragma unroll
+      for (int f = 0; f < 3; ++f) d[f][c] = tp * fmax(d[f][c], 0.0) - tn * fmax(-d[f][c], 0.0);
+      const double lin = (c_linb[j * 3 + 0] * d[0][c] + c_linb[j * 3 + 1] * d[1][c]) + c_linb[j * 3 + 2] * d[2][c];
+      q[c] = avg[c] + lin;
+    }
+    grp_sync();  // the buffers are free for the group's next element
   }
-  if (troubled) reconstruct_p1(q, avg, sn, a.limiter);
-  const double theta = positivity(q, avg, a.gamma);
-  if (rec) rec[TAPE_THETA] = theta;
-  bool finite = true;
-  double rmin = INF, pmin = INF, lam = -INF;
+  // Zhang-Shu positivity scaling (limiting.py:166-200).
+  const double pbar = pressure_of(avg, gm1);
+  const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
+  double scale = 1.0;
+  const double rlow = grp_min(q[0]);
+  if (rlow < eps) {
+    const double span = avg[0] - rlow;
+    double theta = (avg[0] - eps) / span;
+    theta = fmin(fmax(theta, 0.0), 1.0);
+    scale = theta;
 #pragma unroll
-  for (int i = 0; i < NP; ++i) {
-    store4(a.q + ((int64_t)i * K + e) * 4, q[i]);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
-    rmin = nan_min(rmin, q[i][0]);
-    pmin = nan_min(pmin, pressure_of(q[i], gm1));
-    if (a.want_dt) lam = i == 0 ? wave_speed(q[i], a.gamma) : nan_max(lam, wave_speed(q[i], a.gamma));
+    for (int c = 0; c < 4; ++c) q[c] = avg[c] + theta * (q[c] - avg[c]);
   }
+  const bool need = pressure_of(q, gm1) < eps;
+  if (grp_any(need)) {
+    double lo = 1.0;
+    if (need) {
+      double dev[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) dev[c] = q[c] - avg[c];
+      double l = 0.0, h = 1.0;
+      for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (l + h);
+        double sv[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sv[c] = avg[c] + mid * dev[c];
+        if (pressure_of(sv, gm1) >= eps) {
+          l = mid;
+        } else {
+          h = mid;
+        }
+      }
+      lo = l;
+    }
+    const double theta = grp_min(lo);
+    if (theta < 1.0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) q[c] = avg[c] + theta * (q[c] - avg[c]);
+      scale = scale * theta;
+    }
+  }
+  if (rec && lane == 0) rec[TAPE_THETA] = scale;
+  if (lane < NP) store4(a.q + ((int64_t)j * K + e) * 4, q);
+  bool fin = true;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) fin = fin && isfinite(q[c]);
+  const bool finite = !grp_any(!fin);
+  const double rmin = grp_min(q[0]);
+  const double pmin = grp_min(pressure_of(q, gm1));
   row[COL_RMIN] = nan_min(row[COL_RMIN], rmin);
   row[COL_PMIN] = nan_min(row[COL_PMIN], pmin);
   if (!finite) row[COL_ADM] = 0.0;
-  if (a.want_dt) row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / lam);
+  if (a.want_dt) {
+    const double lam = grp_max(wave_speed(q, a.gamma));
+    row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / lam);
+  }
 }
 
 __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
+  __shared__ double qs[FBLOCK / RG][NP][4];
+  __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
   part_neutral(row);
   const int n = *a.work_count;
-  for (int idx = blockIdx.x * FBLOCK + threadIdx.x; idx < n; idx += gridDim.x * FBLOCK)
-    repair_element(a, a.work[idx], idx, row);
+  const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
+  for (int idx = blockIdx.x * (FBLOCK / RG) + g; idx < n; idx += gridDim.x * (FBLOCK / RG))
+    repair_group(a, a.work[idx], idx, lane, qs[g], ds[g], row);
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)blockIdx.x * PART_STRIDE);
 }
 
 // Batched variant: block b repairs the worklist of 256-element block b, so
 // each case's report rows stay contiguous; it re-arms its list counter.
 __global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
+  __shared__ double qs[FBLOCK / RG][NP][4];
+  __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
   part_neutral(row);
   const int b = blockIdx.x;
@@ -353,7 +472,9 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
   if (n > 0) {
     const int cs = a.bv.lcase[b];
     a.limiter = a.bv.limiter[cs];
-    for (int idx = threadIdx.x; idx < n; idx += FBLOCK) repair_element(a, a.work[(int64_t)b * LBLOCK + idx], -1, row);
+    const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
+    for (int idx = g; idx < n; idx += FBLOCK / RG)
+      repair_group(a, a.work[(int64_t)b * LBLOCK + idx], -1, lane, qs[g], ds[g], row);
   }
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)b * PART_STRIDE);
   if (threadIdx.x == 0) a.bv.wcount[b] = 0;
