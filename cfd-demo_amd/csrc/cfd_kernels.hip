@@ -6,7 +6,7 @@
 //   k_limit_detect troubled-cell detection from the summaries; elements that
 //                  need repair go to a worklist, the rest report directly
 //   k_limit_fix    P1 reconstruction and Zhang-Shu positivity, in place, for
-//                  worklist elements only
+//                  worklist elements only (16 lanes per element)
 //   k_dt           CFL time step of a state (first step / time_step API)
 //   k_finalize     deterministic two-level reduction of per-block partials
 //
@@ -215,6 +215,38 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
 // ---------------------------------------------------------------------------
 // Limiter, pass 1: detection for every element from the stage summaries.
 // ---------------------------------------------------------------------------
+// Detection of element e from its summary row sm (limiting.py:136-151, and the
+// positivity trigger of limiting.py:166-200): troubled flag, report values
+// folded into row, and whether the element goes to the repair pass (entry:
+// e, bit 31 set when troubled).
+__device__ __forceinline__ bool detect_element(const LimitArgs& a, int64_t e, const double* sm,
+                                               double row[PART_COLS], int& entry) {
+  double avg[4];
+  load4(sm, avg);
+  const int fl = (int)sm[SUMM_FLAGS];
+  bool troubled = false;
+  if (a.limit_slopes) {
+    double sn[3][4];
+    troubled = detect_troubled(a, e, avg, sm, sn);
+    if (a.force) troubled = a.force[e] != 0;
+  }
+  const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
+  const double pbar = pressure_of(avg, a.gamma - 1.0);
+  const bool avg_ok = (avg[0] > 0.0) && (pbar > 0.0);
+  if (troubled) row[COL_TROUBLED] += 1.0;
+  if (work) {
+    entry = (int)e | (troubled ? (int)0x80000000u : 0);
+    if (!avg_ok) row[COL_ADM] = 0.0;  // finiteness is checked by the repair pass
+  } else {
+    row[COL_RMIN] = nan_min(row[COL_RMIN], sm[SUMM_RMIN]);
+    row[COL_PMIN] = nan_min(row[COL_PMIN], sm[SUMM_PMIN]);
+    if (!(avg_ok && !(fl & SF_NONFINITE))) row[COL_ADM] = 0.0;
+    if (a.want_dt) row[COL_DT] = nan_min(row[COL_DT], a.dtg[e] / sm[SUMM_LAMBDA]);
+  }
+  if (a.flags) a.flags[e] = troubled ? 1 : 0;
+  return work;
+}
+
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
@@ -244,22 +276,8 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     __syncthreads();
   }
   if (live) {
-    const double* sm = &srow[threadIdx.x][0];
-    double avg[4];
-    load4(sm, avg);
-    const int fl = (int)sm[SUMM_FLAGS];
-    bool troubled = false;
-    if (a.limit_slopes) {
-      double sn[3][4];
-      troubled = detect_troubled(a, e, avg, sm, sn);
-      if (a.force) troubled = a.force[e] != 0;
-    }
-    const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
-    const double pbar = pressure_of(avg, a.gamma - 1.0);
-    const bool avg_ok = (avg[0] > 0.0) && (pbar > 0.0);
-    row[COL_TROUBLED] = troubled ? 1.0 : 0.0;
-    if (work) {
-      const int entry = (int)e | (troubled ? (int)0x80000000u : 0);
+    int entry = 0;
+    if (detect_element(a, e, &srow[threadIdx.x][0], row, entry)) {
       if (batched) {
         const int slot = atomicAdd(&a.bv.wcount[blk], 1);
         a.work[(int64_t)blk * LBLOCK + slot] = entry;
@@ -267,14 +285,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
         const int slot = atomicAdd(a.work_count, 1);
         a.work[slot] = entry;
       }
-      row[COL_ADM] = avg_ok ? 1.0 : 0.0;  // finiteness is checked by k_limit_fix
-    } else {
-      row[COL_RMIN] = sm[SUMM_RMIN];
-      row[COL_PMIN] = sm[SUMM_PMIN];
-      row[COL_ADM] = (avg_ok && !(fl & SF_NONFINITE)) ? 1.0 : 0.0;
-      if (a.want_dt) row[COL_DT] = a.dtg[e] / sm[SUMM_LAMBDA];
     }
-    if (a.flags) a.flags[e] = troubled ? 1 : 0;
   }
   block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
@@ -564,6 +575,12 @@ __global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ 
     const double* p = part + (int64_t)r * PART_STRIDE;
 #pragma unroll
     for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+  }
+  if (gridDim.x == 1) {
+    // Small meshes reduce in one block: its result is the report row.
+    block_reduce_row<LBLOCK>(row, out);
+    if (threadIdx.x == 0 && work_count) *work_count = 0;
+    return;
   }
   block_reduce_row<LBLOCK>(row, chunk + (int64_t)blockIdx.x * PART_STRIDE);
   __threadfence();
