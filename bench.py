@@ -63,10 +63,15 @@ def cpu_baseline(n: int, steps: int) -> dict:
                    ("SE", (cx > .5) & (cy < .5))):
         st[m] = primitive_to_conserved(*QUADRANT_STATES[key], 1.4)
     q = np.broadcast_to(st[None], s.state_shape).copy()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        q, rep = s.step(q, s.time_step(q))
-    el = time.perf_counter() - t0
+    # One thread: the oracle's contractions are np.einsum loops (no BLAS), and
+    # the limit keeps any BLAS call it makes on the one core reported.
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            q, rep = s.step(q, s.time_step(q))
+        el = time.perf_counter() - t0
     dof = 60 * s.mesh.num_elements
     model = "unknown"
     try:
@@ -248,7 +253,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1000, help="rectangles per side (K = 2 n^2)")
     ap.add_argument("--cpu-n", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-cases", type=int, default=256)
