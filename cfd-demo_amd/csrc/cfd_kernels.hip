@@ -7,6 +7,7 @@
 //                  need repair go to a worklist, the rest report directly
 //   k_limit_fix    P1 reconstruction and Zhang-Shu positivity, in place, for
 //                  worklist elements only (16 lanes per element)
+//   k_limit_block  both passes per 256-element block, for small meshes
 //   k_dt           CFL time step of a state (first step / time_step API)
 //   k_finalize     deterministic two-level reduction of per-block partials
 //
@@ -489,6 +490,44 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
   }
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)b * PART_STRIDE);
   if (threadIdx.x == 0) a.bv.wcount[b] = 0;
+}
+
+// Single-case limiter without a tape in one pass: block b detects its
+// LBLOCK elements (as k_limit_detect) and its 16-lane groups then repair the
+// block's own worklist, kept in LDS (as k_limit_fix). A repair writes only its
+// element's nodes and detection reads only summaries, so no grid-wide barrier
+// is needed between the two; one partial row per block.
+__global__ void __launch_bounds__(LBLOCK) k_limit_block(LimitArgs a) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
+  __shared__ double srow[LBLOCK][18];
+  __shared__ int wl[LBLOCK];
+  __shared__ int wn;
+  __shared__ double qs[LBLOCK / RG][NP][4];
+  __shared__ double ds[LBLOCK / RG][12];
+  if (threadIdx.x == 0) wn = 0;
+  {
+    const int64_t r0 = (int64_t)blk * LBLOCK;
+    const int nrow = (int)min((int64_t)LBLOCK, a.K - r0);
+    const double2* src = reinterpret_cast<const double2*>(a.summ + r0 * SUMM_STRIDE);
+    for (int i = threadIdx.x; i < nrow * (SUMM_STRIDE / 2); i += LBLOCK) {
+      const double2 v = src[i];
+      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2))] = v.x;
+      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2)) + 1] = v.y;
+    }
+    __syncthreads();
+  }
+  if (e < a.K) {
+    int entry = 0;
+    if (detect_element(a, e, &srow[threadIdx.x][0], row, entry)) wl[atomicAdd(&wn, 1)] = entry;
+  }
+  __syncthreads();
+  const int n = wn;
+  const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
+  for (int idx = g; idx < n; idx += LBLOCK / RG) repair_group(a, wl[idx], -1, lane, qs[g], ds[g], row);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
 // ---------------------------------------------------------------------------
