@@ -7,7 +7,7 @@ namespace cfd {
 
 constexpr int LBLOCK = 256;      // threads per block of the per-element passes
 constexpr int FBLOCK = 64;       // threads per block of the limiter repair pass
-constexpr int FGRID = 1024;      // blocks of the limiter repair pass
+constexpr int FGRID = 4096;      // blocks of the limiter repair pass (4 elements in flight each)
 constexpr int SUMM_STRIDE = 16;  // doubles of limiter summary per element
 constexpr int PART_STRIDE = 16;  // doubles per partial row (PART_COLS used)
 constexpr int PART_COLS = 9;

@@ -466,6 +466,13 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
   double row[PART_COLS];
   part_neutral(row);
   const int n = *a.work_count;
+  if ((int)blockIdx.x * (FBLOCK / RG) >= n) {
+    // No worklist entry reaches this block: its partial row is the neutral one.
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < PART_COLS; ++c) a.part[(int64_t)blockIdx.x * PART_STRIDE + c] = row[c];
+    return;
+  }
   const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
   for (int idx = blockIdx.x * (FBLOCK / RG) + g; idx < n; idx += gridDim.x * (FBLOCK / RG))
     repair_group(a, a.work[idx], idx, lane, qs[g], ds[g], row);
