@@ -48,6 +48,40 @@ def test_batched_matches_single_case_runs(use_graph, quad, per_flux, monkeypatch
     batch.close()
 
 
+@pytest.mark.parametrize("per_flux", [False, True])
+def test_batched_every_flux_and_limiter(per_flux, monkeypatch):
+    """All 15 flux x limiter pairs in one batch (van_leer and unlimited
+    included, which the seeded benchmark tubes do not cycle through), under
+    the run-time-flux and the one-launch-per-flux stage, against the
+    single-case runs. Mild tubes (density and pressure ratios about 2): strong
+    random tubes can reach the positivity floor, where the CFL step collapses
+    and last-bit differences change the step count (see the exact-solution
+    test below)."""
+    fluxes = ("rusanov", "roe", "hllc")
+    limiters = ("minmod", "mc", "van_leer", "superbee", "unlimited")
+    specs = []
+    for i in range(15):
+        nx = 50 + 2 * i
+        specs.append(dict(nx=nx, ny=max(2, nx // 10), left=(1.0 + 0.02 * i, 0.1 * (i % 2), 0.0, 1.0),
+                          right=(0.5, -0.05 * (i % 3), 0.0, 0.4 + 0.02 * i), x0=0.45 + 0.01 * (i % 5),
+                          final_time=0.01, flux=fluxes[i % 3], limiter=limiters[i // 3]))
+    cases = [shock_tube_case(sp) for sp in specs]
+    print(f"case batched 15 flux x limiter pairs per_flux={per_flux}", flush=True)
+    monkeypatch.setenv("CFDHIP_STAGE_PER_FLUX", "1" if per_flux else "0")
+    batch = BatchedSolver(cases)
+    monkeypatch.delenv("CFDHIP_STAGE_PER_FLUX")
+    got = batch.run()
+    for sp, case, r in zip(specs, cases, got):
+        ref = case.solver.run(case.initial_state, case.final_time)
+        what = (sp["flux"], sp["limiter"])
+        assert r.steps == ref.steps and r.rejected_steps == ref.rejected_steps, what
+        assert r.time == ref.time == case.final_time, what
+        assert r.max_troubled_cells == ref.max_troubled_cells, what
+        assert np.abs(r.state - ref.state).max() <= 1e-13 * np.abs(ref.state).max(), what
+        assert np.allclose(r.boundary_flux_integral, ref.boundary_flux_integral, rtol=1e-12, atol=1e-15), what
+    batch.close()
+
+
 def test_autotuner_and_rejection_case():
     cases = _cases(3, seed=5, final_time=0.003)
     bad_spec = random_shock_tube_specs(1, seed=9, nx_range=(50, 50))[0]
