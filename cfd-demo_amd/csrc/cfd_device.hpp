@@ -128,6 +128,12 @@ struct LimitArgs {
   // Replay (nullptr: off): the troubled decision of element e is force[e]
   // instead of the detector's (frozen-decision forward for the adjoint).
   const uint8_t* force = nullptr;
+  // Stage-kernel partial rows (nullptr: none): detection block b folds rows
+  // [b * stage_per_blk, (b + 1) * stage_per_blk) of the n_stage_rows into its
+  // own row, so the report reduction reads one row per LBLOCK elements.
+  const double* stage_rows = nullptr;
+  int stage_per_blk = 0;
+  int n_stage_rows = 0;
 };
 
 // Tape record of one limiter worklist element (TAPE_STRIDE doubles):

@@ -248,6 +248,19 @@ __device__ __forceinline__ bool detect_element(const LimitArgs& a, int64_t e, co
   return work;
 }
 
+// The stage rows of detection block blk (lanes 0 .. stage_per_blk - 1 take
+// one each).
+__device__ __forceinline__ void fold_stage_rows(const LimitArgs& a, int blk, double row[PART_COLS]) {
+  if (a.stage_rows && (int)threadIdx.x < a.stage_per_blk) {
+    const int r = blk * a.stage_per_blk + (int)threadIdx.x;
+    if (r < a.n_stage_rows) {
+      const double* p = a.stage_rows + (int64_t)r * PART_STRIDE;
+#pragma unroll
+      for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
@@ -288,6 +301,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
       }
     }
   }
+  fold_stage_rows(a, blk, row);
   block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
@@ -534,6 +548,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_block(LimitArgs a) {
   const int n = wn;
   const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
   for (int idx = g; idx < n; idx += LBLOCK / RG) repair_group(a, wl[idx], -1, lane, qs[g], ds[g], row);
+  fold_stage_rows(a, blk, row);
   block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
