@@ -5,6 +5,7 @@
 //                  in the last stage the CFL wave speed)
 //   k_limit_detect troubled-cell detection from the summaries; elements that
 //                  need repair go to a worklist, the rest report directly
+//                  (k_limit_detect_direct: the same without slope limiting)
 //   k_limit_fix    P1 reconstruction and Zhang-Shu positivity, in place, for
 //                  worklist elements only (16 lanes per element)
 //   k_limit_block  both passes per 256-element block, for small meshes
@@ -299,6 +300,25 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
         const int slot = atomicAdd(a.work_count, 1);
         a.work[slot] = entry;
       }
+    }
+  }
+  fold_stage_rows(a, blk, row);
+  block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
+}
+
+// Unlimited runs (no slopes, so no neighbour rows): each lane reads its own
+// summary row straight from memory, without the LDS staging that caps
+// k_limit_detect's occupancy.
+__global__ void __launch_bounds__(LBLOCK) k_limit_detect_direct(LimitArgs a) {
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
+  double row[PART_COLS];
+  part_neutral(row);
+  if (e < a.K) {
+    int entry = 0;
+    if (detect_element(a, e, a.summ + e * SUMM_STRIDE, row, entry)) {
+      const int slot = atomicAdd(a.work_count, 1);
+      a.work[slot] = entry;
     }
   }
   fold_stage_rows(a, blk, row);
