@@ -75,6 +75,32 @@ def test_limit_matches_oracle(limiter):
     assert np.allclose(avg_gpu, o.cell_averages(q), rtol=1e-13, atol=1e-14)
 
 
+@pytest.mark.parametrize("name,limit_slopes", [("mc", True), ("superbee", True), ("minmod", False)])
+def test_apply_limiters_matches_oracle(name, limit_slopes):
+    """The pure-function limiter entry point over a solver's tables, as the
+    reference's tests call it (tests/unit/test_limiting_and_time.py:85-87):
+    limited state, averages of the input and the troubled count (int32 zero
+    without slope limiting) against the oracle's."""
+    import cfd_demo_amd as gpu
+    from oracle import physics as ophys
+    from oracle.limiting import apply_limiters as oracle_apply
+
+    g, o = _pair("sod", limiter=name)
+    rng = np.random.default_rng(7)
+    x, y = g.node_coordinates()
+    q = smooth_state(x, y) * (1.0 + 0.3 * rng.standard_normal(g.state_shape))
+    print(f"case apply_limiters phi={name} limit_slopes={limit_slopes}", flush=True)
+    q_gpu, avg_gpu, cnt_gpu = gpu.apply_limiters(q, g.limiter_operators, gpu.get_limiter_function(name), 1.4,
+                                                 limit_slopes=limit_slopes)
+    q_ref, avg_ref, cnt_ref = oracle_apply(q, o.limiter_operators, ophys.get_limiter_function(name), 1.4,
+                                           limit_slopes)
+    assert rel_err(q_gpu, q_ref) <= 1e-12
+    assert np.allclose(avg_gpu, avg_ref, rtol=1e-13, atol=1e-14)
+    assert int(cnt_gpu) == int(cnt_ref)
+    if not limit_slopes:
+        assert cnt_gpu.dtype == np.int32 and int(cnt_gpu) == 0
+
+
 def test_positivity_repair_matches_oracle():
     g, o = _pair("fixed", limiter="unlimited", nx=4, ny=4)
     q = np.broadcast_to(np.array([1.0, 0.0, 0.0, 2.5]), g.state_shape).copy()
