@@ -1,8 +1,9 @@
 """GPU edge cases through the C ABI against the oracle: the smallest mesh
 (one rectangle, two elements, every face on the boundary), element counts at
 and around the stage-kernel and limiter workgroup sizes (full and partial last
-workgroups), and the context's rejection of malformed tables with the
-reference's exception class (ValueError) and a message naming the problem.
+workgroups), the context's rejection of malformed tables with the
+reference's exception class (ValueError) and a message naming the problem,
+and the two limiter launch paths giving bit-identical runs.
 """
 
 import ctypes as C
@@ -109,3 +110,25 @@ def test_malformed_tables_are_rejected():
         print(f"case malformed: {msg}", flush=True)
         with pytest.raises(ValueError, match=msg):
             _lib.check(create(*args, **kw))
+
+
+def test_limiter_paths_bit_identical(monkeypatch):
+    """The one-launch limiter (detection and repair per block) and the
+    two-launch one give bit-identical runs: same states, step counts, troubled
+    maxima, minima and boundary-flux integral. Sod tube 250 x 10 (5,000
+    elements, above the one-launch threshold), HLLC + MC, forced both ways."""
+    from cfd_demo_amd.cases import sod_shock_tube
+
+    runs = []
+    for onepass in ("1", "0"):
+        monkeypatch.setenv("CFDHIP_LIMIT_ONEPASS", onepass)
+        case = sod_shock_tube(250, 10, flux="hllc", limiter="mc", cfl=0.5)
+        print(f"case limiter paths onepass={onepass} K={case.solver.mesh.num_elements}", flush=True)
+        runs.append(case.solver.run(case.initial_state, 0.02))
+        case.solver.close()
+    a, b = runs
+    assert a.steps == b.steps and a.rejected_steps == b.rejected_steps
+    assert a.max_troubled_cells == b.max_troubled_cells > 0
+    assert np.array_equal(a.state, b.state)
+    assert a.min_density == b.min_density and a.min_pressure == b.min_pressure
+    assert np.array_equal(a.boundary_flux_integral, b.boundary_flux_integral)
