@@ -262,6 +262,40 @@ __device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], do
   }
 }
 
+// The stage kernels' partial row: only the four boundary-flux sums vary, the
+// other columns are neutral. Same butterfly and wave order as
+// block_reduce_row on those columns (identical bits), without the work on the
+// constant ones.
+template <int NT>
+__device__ __forceinline__ void block_reduce_bflux(const double bf[4], double* out) {
+  constexpr int NW = NT / 64;
+  __shared__ double sh[NW][4];
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v[c] = bf[c];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] = v[c] + __shfl_xor(v[c], off, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sh[w][c] = v[c];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double row[PART_COLS];
+    part_neutral(row);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double t = sh[0][c];
+      for (int k = 1; k < NW; ++k) t = t + sh[k][c];
+      row[c] = t;
+    }
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) out[c] = row[c];
+  }
+}
+
 // sqrt(u^2 + v^2) + c with the reference's operation order
 // (time_integration.py:39-40); contraction off so k_dt and the fused stage
 // summary give the same bits as each other.

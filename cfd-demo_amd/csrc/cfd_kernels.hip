@@ -205,13 +205,7 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
   if constexpr (MODE != MODE_RHS) {
     if (live) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ + e * SUMM_STRIDE);
   }
-  if constexpr (MODE != MODE_SUMMARY) {
-    double row[PART_COLS];
-    part_neutral(row);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) row[c] = bfl[c];
-    block_reduce_row<SB>(row, a.part + (int64_t)blk * PART_STRIDE);
-  }
+  if constexpr (MODE != MODE_SUMMARY) block_reduce_bflux<SB>(bfl, a.part + (int64_t)blk * PART_STRIDE);
 }
 
 // ---------------------------------------------------------------------------
