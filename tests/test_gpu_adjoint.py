@@ -115,6 +115,26 @@ def test_march_adjoint_tape_regrows():
     run.close()
 
 
+def test_repeated_backward_and_forward_agree():
+    """The forward records its last segment for the backward; a second
+    backward (after the first reused the segment buffers) and a new
+    forward-backward pair give the same gradient bit for bit."""
+    torch = _torch()
+    g, o = _pair("hllc", "minmod", "periodic", n=4, seed=14)
+    x, y = o.node_coordinates()
+    q0 = torch.from_numpy(smooth_state(x, y, amp=0.3)).cuda()
+    dts = np.full(5, 0.3 * o.time_step(q0.cpu().numpy()))
+    lam = torch.from_numpy(np.random.default_rng(15).standard_normal(q0.shape)).cuda()
+    run = adj.DiscreteAdjoint(g, len(dts), dts=dts, checkpoint_every=2)
+    run.forward(q0)
+    g1 = run.backward(lam).cpu().numpy()
+    g2 = run.backward(lam).cpu().numpy()
+    run.forward(q0)
+    g3 = run.backward(lam).cpu().numpy()
+    assert np.array_equal(g1, g2) and np.array_equal(g1, g3)
+    run.close()
+
+
 def test_batched_adjoint_matches_per_field():
     torch = _torch()
     g, o = _pair("roe", "minmod", "periodic", n=4, seed=14)
