@@ -8,7 +8,7 @@ namespace cfd {
 constexpr int LBLOCK = 256;      // threads per block of the per-element passes
 constexpr int FBLOCK = 64;       // threads per block of the limiter repair pass
 constexpr int FGRID = 4096;      // blocks of the limiter repair pass (4 elements in flight each)
-constexpr int SUMM_STRIDE = 16;  // doubles of limiter summary per element
+constexpr int SUMM_STRIDE = 14;  // doubles of limiter summary per element (7 double2 pieces)
 constexpr int PART_STRIDE = 16;  // doubles per partial row (PART_COLS used)
 constexpr int PART_COLS = 9;
 constexpr int GEO_STRIDE = 16;   // doubles of geometry per element
@@ -358,7 +358,6 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
   o[4] = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
   o[5] = make_double2(rmin, pmin);
   o[6] = make_double2((double)flags, lam);
-  o[7] = make_double2(0.0, 0.0);
 }
 
 // _minmod (limiting.py:124-125).

@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     a.limit_slopes = a.limiter != LIM_UNLIMITED;
   }
   // The block's summary rows through LDS: contiguous 16-byte loads across the
-  // block instead of one 128-byte row per lane (rows padded to 18 doubles so
+  // block instead of one 112-byte row per lane (rows padded to 18 doubles so
   // the per-row reads below are free of bank conflicts).
   __shared__ double srow[LBLOCK][18];
   {
