@@ -12,7 +12,8 @@ constexpr int SUMM_STRIDE = 14;  // doubles of limiter summary per element (7 do
 constexpr int PART_STRIDE = 16;  // doubles per partial row (PART_COLS used)
 constexpr int PART_COLS = 9;
 constexpr int GEO_STRIDE = 16;   // doubles of geometry per element
-constexpr int LIM_STRIDE = 8;    // doubles of limiter geometry per element
+constexpr int LIM_STRIDE = 8;    // doubles of limiter geometry per element in the host tables
+constexpr int LIM_DEV = 6;       // ... on the device: alpha[3], beta[3] (dt geometry and J go to their own arrays)
 constexpr double INF = __builtin_huge_val();
 
 // Summary layout (per element, written by k_stage, read by k_limit).
@@ -378,7 +379,7 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
   const int4 cn = reinterpret_cast<const int4*>(a.conn)[e];
   const int nbr[3] = {cn.x, cn.y, cn.z};
   const uint32_t code = (uint32_t)cn.w;
-  const double* lim = a.lim + e * LIM_STRIDE;
+  const double* lim = a.lim + e * LIM_DEV;
   const double* geo = a.geo + e * GEO_STRIDE;
   // Neighbour-average differences per face, kept as three separate arrays so
   // the partner-face selection below stays a register select.
