@@ -135,6 +135,9 @@ struct LimitArgs {
   const double* stage_rows = nullptr;
   int stage_per_blk = 0;
   int n_stage_rows = 0;
+  // Repair pass (nullptr: off): the worklist length of this pass is stored
+  // here (the adjoint's recomputation skips passes that repaired nothing).
+  int* count_out = nullptr;
 };
 
 // Tape record of one limiter worklist element (TAPE_STRIDE doubles):

@@ -494,6 +494,7 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
   double row[PART_COLS];
   part_neutral(row);
   const int n = *a.work_count;
+  if (a.count_out && blockIdx.x == 0 && threadIdx.x == 0) *a.count_out = n;
   if ((int)blockIdx.x * (FBLOCK / RG) >= n) {
     // No worklist entry reaches this block: its partial row is the neutral one.
     if (threadIdx.x == 0)

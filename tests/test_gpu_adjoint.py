@@ -94,6 +94,27 @@ def test_march_adjoint_matches_autograd(flux, limiter, bcs):
     run.close()
 
 
+@pytest.mark.parametrize("flux,bcs", [("rusanov", "periodic"), ("hllc", "sod")])
+def test_march_adjoint_two_launch_limiter_matches_autograd(flux, bcs, monkeypatch):
+    """Unlimited marches with the two-launch limiter (as on meshes above 4,096
+    elements): the forward records each pass's repair count and the backward's
+    recomputation skips the passes that repaired nothing."""
+    torch = _torch()
+    monkeypatch.setenv("CFDHIP_LIMIT_ONEPASS", "0")
+    g, o = _pair(flux, "unlimited", bcs, n=4, seed=16)
+    x, y = o.node_coordinates()
+    q0 = smooth_state(x, y, amp=0.3)
+    dts = np.full(5, 0.3 * o.time_step(q0))
+    lam = np.random.default_rng(17).standard_normal(q0.shape)
+    want, qn_oracle = at.march_vjp(o, q0, dts, lam)
+    run = adj.DiscreteAdjoint(g, len(dts), dts=dts, checkpoint_every=2)
+    qn = run.forward(torch.from_numpy(q0).cuda())
+    assert rel_err(qn.cpu().numpy(), qn_oracle) <= 1e-12
+    got = run.backward(torch.from_numpy(lam).cuda()).cpu().numpy()
+    assert _vrel(got, want) <= 1e-10, (flux, bcs, _vrel(got, want))
+    run.close()
+
+
 def test_march_adjoint_tape_regrows():
     torch = _torch()
     os.environ["CFDHIP_ADJOINT_TAPE_RECORDS"] = "4"
