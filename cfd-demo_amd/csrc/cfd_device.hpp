@@ -321,6 +321,16 @@ __device__ __forceinline__ void pressure_and_speed(const double q[4], double gam
   ws = sqrt(u * u + v * v) + sqrt(gamma * p / q[0]);
 }
 
+// A lower bound of the correctly rounded pressure (pressure_of) from the
+// pressure p = gm1 * (E - ke) computed with the refined reciprocal and
+// possibly contracted into FMAs. The two differ by a few ulps of the terms
+// that cancel, |E| and ke (the reciprocal, the products, the sum and the
+// difference each add at most a few half-ulps); 2^-47 of (|E| + |ke|) is 64
+// half-ulps, so p minus that never exceeds the correctly rounded value.
+__device__ __forceinline__ double pressure_floor(double p, double E, double ke, double gm1) {
+  return p - gm1 * (fabs(E) + fabs(ke)) * 0x1p-47;
+}
+
 // Per-element limiter summary of a stage state held in registers.
 template <bool WITH_LAMBDA>
 __device__ __forceinline__ void write_summary(const double q[NP][NV], double gamma, double* sm) {
@@ -334,22 +344,23 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
   const double eps = nan_min(1e-13, nan_min(avg[0], pbar));
   // Nodal pressures and wave speeds with the refined hardware reciprocal and
   // square root. need_p only sends an element to the repair pass, which
-  // decides with correctly rounded pressures, so it errs towards sending.
-  const double eps_send = eps + fabs(eps) * 1e-12;
+  // decides with correctly rounded pressures; it tests a lower bound of the
+  // correctly rounded pressure, so no node below epsilon is missed.
   double rmin = INF, pmin = INF, lam = -INF;
   bool finite = true, need_p = false;
 #pragma unroll
   for (int i = 0; i < NP; ++i) {
     const double inv = frcp(q[i][0]);
     const double u = q[i][1] * inv, v = q[i][2] * inv;
-    const double p = gm1 * (q[i][3] - 0.5 * (q[i][1] * u + q[i][2] * v));
+    const double ke = 0.5 * (q[i][1] * u + q[i][2] * v);
+    const double p = gm1 * (q[i][3] - ke);
     if constexpr (WITH_LAMBDA) {
       const double ws = fsqrt(u * u + v * v) + fsqrt(gamma * p * inv);
       lam = i == 0 ? ws : nan_max(lam, ws);
     }
     rmin = nan_min(rmin, q[i][0]);
     pmin = nan_min(pmin, p);
-    need_p = need_p || (p < eps_send);
+    need_p = need_p || (pressure_floor(p, q[i][3], ke, gm1) < eps);
 #pragma unroll
     for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
   }

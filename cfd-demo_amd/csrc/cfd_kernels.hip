@@ -17,15 +17,14 @@
 // __constant__ memory and are read with wave-uniform indices, i.e. as scalar
 // FMA operands.
 //
-// Three stage-kernel layouts: k_stage4 (cfd_stage4.inc; four lanes per
+// Two stage-kernel layouts: k_stage4 (cfd_stage4.inc; four lanes per
 // element, 15 accumulators per lane, three waves per SIMD) is the default for
 // a fixed flux; k_stage below (one lane per element, 60 accumulators, two
 // waves per SIMD) serves the run-time flux of batched solves, the
-// summary-only mode and other workgroup sizes; k_stagem (cfd_stage_mfma.inc)
-// runs the contractions on FP64 MFMA and is opt-in (slower than k_stage4 on
-// MI355X, whose FP64 matrix and vector rates are equal). A
-// 16-lanes-per-element MFMA tile variant and an LDS-staged
-// one-lane-per-element variant were also measured slower and are not kept.
+// summary-only mode and other workgroup sizes. Contractions on FP64 MFMA
+// (in two lane layouts) and an LDS-staged one-lane-per-element variant were
+// measured slower on MI355X, whose FP64 matrix and vector rates are equal,
+// and are not kept.
 #include "cfd_device.hpp"
 
 namespace cfd {
@@ -705,7 +704,6 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 }  // namespace cfd
 
 #include "cfd_stage4.inc"
-#include "cfd_stage_mfma.inc"
 #include "cfd_adjoint_kernels.inc"
 #include "cfd_capi.inc"
 #include "cfd_batched.inc"

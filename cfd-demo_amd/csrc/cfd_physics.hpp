@@ -34,7 +34,12 @@ constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.
 // residual correction; sqrt(x) from v_rsq_f64 refined by a coupled
 // Newton step and a residual correction. Each is within a few ulps of the
 // correctly rounded result for the positive, normal arguments the fluxes
-// produce (checked against NumPy in the pointwise-flux tests).
+// produce (checked against NumPy in the pointwise-flux tests). fsqrt(+0) is
+// +0: the seed is the reciprocal square root of x plus the smallest
+// subnormal, which is x itself for every x above 2^-1020 (the addend is below
+// half an ulp) and keeps the seed finite at x = 0, so x * seed is 0 instead of
+// 0 * inf = NaN; the Newton steps use x itself. Negative and NaN arguments
+// still give NaN.
 __device__ __forceinline__ double frcp(double b) {
   double r = __builtin_amdgcn_rcp(b);
   double e = fma(-b, r, 1.0);
@@ -48,7 +53,7 @@ __device__ __forceinline__ double fdiv(double a, double b) {
   return fma(fma(-b, q, a), r, q);
 }
 __device__ __forceinline__ double fsqrt(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
+  const double y = __builtin_amdgcn_rsq(x + 4.9406564584124654e-324);
   double h = 0.5 * y;
   double g = x * y;
   const double r = fma(-g, h, 0.5);

@@ -113,9 +113,9 @@ int cfd_ctx_create(const CfdMeshTables* mesh, int flux_id, int limiter_id, doubl
                    int max_rejections, CfdCtx** out);
 int cfd_ctx_destroy(CfdCtx* ctx);
 /* Tuning: stage block 64 (default, or the CFDHIP_STAGE_BLOCK environment
- * variable) runs the four-lane stage kernel (32 elements per workgroup), or
- * the MFMA kernel when enabled; 128 or 256 run the one-lane kernel with that
- * many elements per workgroup. No reference counterpart. */
+ * variable) runs the four-lane stage kernel (32 elements per workgroup);
+ * 128 or 256 run the one-lane kernel with that many elements per workgroup.
+ * No reference counterpart. */
 int cfd_ctx_set_stage_block(CfdCtx* ctx, int elements_per_workgroup);
 
 /* _rhs = jit(dg_rhs) (solver.py:118, dg.py:140-165): rhs (15,K,4), bflux (4). */

@@ -18,15 +18,14 @@ FLUXES = ["rusanov", "roe", "hllc"]
 BCS = {"sod": sod_bcs, "fixed": fixed_all, "trans": trans_all, "mixed": mixed_bcs, "periodic": periodic_bcs}
 
 
-@pytest.fixture(autouse=True, params=["quad", "mfma", "lane", "quad-split"])
+@pytest.fixture(autouse=True, params=["quad", "lane", "quad-split"])
 def stage_layout(request, monkeypatch):
     """Every parity test under each stage-kernel layout: four lanes per
-    element (the default), the MFMA contractions, one lane per element; and
-    the default layout with the limiter's detection and repair as two launches
-    (what meshes above 4,096 elements use) instead of the one-pass kernel the
-    small meshes here get."""
+    element (the default), one lane per element; and the default layout with
+    the limiter's detection and repair as two launches (what meshes above
+    4,096 elements use) instead of the one-pass kernel the small meshes here
+    get."""
     monkeypatch.setenv("CFDHIP_STAGE_QUAD", "0" if request.param == "lane" else "1")
-    monkeypatch.setenv("CFDHIP_STAGE_MFMA", "1" if request.param == "mfma" else "0")
     monkeypatch.setenv("CFDHIP_LIMIT_ONEPASS", "0" if request.param == "quad-split" else "1")
     print(f"layout {request.param}", flush=True)
     return request.param
