@@ -60,6 +60,13 @@ class CfdAdjointStats(C.Structure):
                 ("device_bytes", C.c_int64)]
 
 
+class CfdDeviceMesh(C.Structure):
+    _fields_ = [("num_elements", C.c_int64), ("geo", C.c_void_p), ("conn", C.c_void_p), ("alpha_beta", C.c_void_p),
+                ("dt_geometry", C.c_void_p), ("bc_states", C.c_void_p), ("num_bc_states", C.c_int32)]
+
+
+CFD_STATIC_MAX_ELEMENTS = 65536
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "cfd_last_error": (C.c_char_p, []),
@@ -75,6 +82,8 @@ SIGNATURES = {
     "cfd_time_step": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cfd_step": (C.c_int, [C.c_void_p, _dp, C.c_double, _dp, C.POINTER(CfdStepReport), _u8p]),
     "cfd_step_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.POINTER(CfdStepReport)]),
+    "cfd_step_device_static": (C.c_int, [C.POINTER(CfdDeviceMesh), C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                         C.c_double, C.c_void_p, C.POINTER(CfdStepReport)]),
     "cfd_cell_averages": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cfd_integrate": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cfd_run": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int64, _dp, C.POINTER(CfdRunResult)]),

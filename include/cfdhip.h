@@ -135,6 +135,35 @@ int cfd_time_step(CfdCtx* ctx, const double* q, double* dt);
 int cfd_step(CfdCtx* ctx, const double* q, double dt, double* q_out, CfdStepReport* report, uint8_t* flags);
 int cfd_step_device(CfdCtx* ctx, const double* q_dev, double dt, double* q_out_dev, CfdStepReport* report);
 
+/* Allocation-free SSP-RK3 step. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same step,
+ * kernels and bits as cfd_step_device, without a context: the tables are the
+ * caller's device arrays, and every other buffer (the intermediate stage
+ * state, limiter summaries, worklist, partial rows, counters, report row)
+ * lives in one static device array of the library, about 40 MB per GPU,
+ * sized for at most CFD_STATIC_MAX_ELEMENTS elements; larger meshes are
+ * rejected with CFD_EINVAL. cfd_init must have been called on the current
+ * device. Runs on the null stream and synchronises before returning. All
+ * calls on one GPU share the array: calls that overlap on one GPU (several
+ * host threads) race and must be serialised by the caller.
+ *  geo (K x 16), conn (K x 4 int32): as in CfdMeshTables (conn is not
+ *       checked here; use tables cfd_ctx_create accepts);
+ *  alpha_beta (K x 6): alpha0..2 | beta0..2 (the first six lim columns);
+ *  dt_geometry (K): lim column 6;  bc_states (num_bc_states x 4). */
+#define CFD_STATIC_MAX_ELEMENTS 65536
+typedef struct {
+  int64_t num_elements;
+  const double* geo;
+  const int32_t* conn;
+  const double* alpha_beta;
+  const double* dt_geometry;
+  const double* bc_states;
+  int32_t num_bc_states;
+} CfdDeviceMesh;
+int cfd_step_device_static(const CfdDeviceMesh* mesh, int flux_id, int limiter_id, double gamma,
+                           const double* q_dev, double dt, double* q_out_dev, CfdStepReport* report);
+
 /* cell_averages (dg.py:168-171) -> (K,4); domain_integral (dg.py:174-176) -> (4). */
 int cfd_cell_averages(CfdCtx* ctx, const double* q, double* averages);
 int cfd_integrate(CfdCtx* ctx, const double* q, double* integral);
