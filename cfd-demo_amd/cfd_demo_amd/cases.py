@@ -3,7 +3,10 @@
 ``sod_shock_tube`` keeps ``src/cfd_demo/cases.py:15-44`` (constants, mesh,
 fixed-state ends, slip walls, per-element centroid initialisation). The other
 constructors build the four-quadrant Riemann problem, the isentropic vortex,
-the seeded random shock tubes and the vortex-plus-bumps fields.
+the seeded random shock tubes, the vortex-plus-bumps fields and the steady
+oblique shock over a compression corner that the reference proposes in
+docs/adr/0011-oblique-shock-verification.md (on the sheared mesh and
+far-field condition of its ADR 0012).
 """
 
 from __future__ import annotations
@@ -12,8 +15,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .boundary import FixedState, Periodic, SlipWall, Transmissive
-from .mesh import rectangle_mesh
+from .boundary import ExactState, FixedState, Periodic, SlipWall, Transmissive
+from .mesh import rectangle_mesh, sheared_rectangle_mesh
 from .physics import primitive_to_conserved
 from .solver import DEFAULT_CFL, EulerSolver, SolverConfig
 
@@ -142,3 +145,126 @@ def vortex_bumps(n: int = 200, *, n_bumps: int = 8, seed: int = 0, amplitude: fl
     x, y = solver.node_coordinates()
     state = vortex_bumps_state(x, y, n_bumps=n_bumps, seed=seed, amplitude=amplitude, width=width, gamma=gamma)
     return Case(solver, state, final_time)
+
+
+# ---------------------------------------------------------------------------
+# Steady oblique shock on an inclined wall (ADR 0011, proposed in the reference)
+# ---------------------------------------------------------------------------
+def shock_deflection(mach: float, beta, gamma: float = 1.4):
+    """Flow deflection of an oblique shock at angle beta (the theta-beta-M
+    relation): tan(theta) = 2 cot(beta) (M^2 sin^2 beta - 1) /
+    (M^2 (gamma + cos 2 beta) + 2)."""
+    m2 = mach * mach
+    return np.arctan(2.0 / np.tan(beta) * (m2 * np.sin(beta) ** 2 - 1.0) / (m2 * (gamma + np.cos(2.0 * beta)) + 2.0))
+
+
+def max_deflection(mach: float, gamma: float = 1.4) -> tuple[float, float]:
+    """(theta_max, beta at theta_max): the largest deflection an attached
+    shock turns a Mach ``mach`` flow by (golden-section search on beta)."""
+    if not mach > 1.0:
+        raise ValueError("The inflow Mach number must exceed 1.")
+    lo, hi = np.arcsin(1.0 / mach), 0.5 * np.pi
+    g = 0.5 * (np.sqrt(5.0) - 1.0)
+    for _ in range(200):
+        a, b = hi - g * (hi - lo), lo + g * (hi - lo)
+        if shock_deflection(mach, a, gamma) < shock_deflection(mach, b, gamma):
+            lo = a
+        else:
+            hi = b
+    beta = 0.5 * (lo + hi)
+    return float(shock_deflection(mach, beta, gamma)), float(beta)
+
+
+def weak_shock_angle(mach: float, theta: float, gamma: float = 1.4) -> float:
+    """The weak-shock angle beta for a wall deflection 0 < theta <
+    theta_max(mach) (bisection on the increasing branch beta in (mu, beta_max))."""
+    theta_max, beta_max = max_deflection(mach, gamma)
+    if not 0.0 < theta < theta_max:
+        raise ValueError(f"The wall angle must lie in (0, {theta_max:.6f}) rad for an attached shock at Mach {mach}.")
+    lo, hi = np.arcsin(1.0 / mach), beta_max
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if shock_deflection(mach, mid, gamma) < theta:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+@dataclass(frozen=True, eq=False)
+class ObliqueShock:
+    """The exact steady solution: uniform upstream state (1, M, 0, 1/gamma)
+    (sound speed 1) above the shock line y = x tan(beta) from the corner at
+    the origin, and the uniform post-shock state below it, moving parallel to
+    the wall (normal-shock relations on M sin(beta))."""
+
+    mach: float
+    theta: float
+    gamma: float
+    beta: float
+    upstream: tuple
+    post: tuple
+
+    def state(self, x, y):
+        """Conserved exact state (..., 4) at physical points."""
+        below = np.asarray(y) <= np.asarray(x) * np.tan(self.beta)
+        up = primitive_to_conserved(*self.upstream, self.gamma)
+        dn = primitive_to_conserved(*self.post, self.gamma)
+        return np.where(below[..., None], dn, up)
+
+    def post_fraction(self, corners) -> np.ndarray:
+        """Area fraction of each triangle (n, 3, 2) below the shock line."""
+        t = np.tan(self.beta)
+        out = np.empty(len(corners))
+        for i, tri in enumerate(np.asarray(corners, dtype=np.float64)):
+            d = tri[:, 1] - tri[:, 0] * t  # <= 0 below the line
+            keep = []
+            for a in range(3):
+                b = (a + 1) % 3
+                if d[a] <= 0.0:
+                    keep.append(tri[a])
+                if (d[a] <= 0.0) != (d[b] <= 0.0):
+                    keep.append(tri[a] + d[a] / (d[a] - d[b]) * (tri[b] - tri[a]))
+            area = 0.0
+            if len(keep) >= 3:
+                p = np.array(keep)
+                area = 0.5 * abs(np.dot(p[:, 0], np.roll(p[:, 1], -1)) - np.dot(p[:, 1], np.roll(p[:, 0], -1)))
+            full = 0.5 * abs((tri[1, 0] - tri[0, 0]) * (tri[2, 1] - tri[0, 1])
+                             - (tri[2, 0] - tri[0, 0]) * (tri[1, 1] - tri[0, 1]))
+            out[i] = min(1.0, area / full)
+        return out
+
+    def average(self, corners) -> np.ndarray:
+        """Exact cell averages (n, 4) of the piecewise-constant solution."""
+        f = self.post_fraction(corners)[:, None]
+        up = primitive_to_conserved(*self.upstream, self.gamma)
+        dn = primitive_to_conserved(*self.post, self.gamma)
+        return f * np.asarray(dn) + (1.0 - f) * np.asarray(up)
+
+
+def oblique_shock_solution(mach: float = 2.0, theta: float = np.radians(10.0), gamma: float = 1.4) -> ObliqueShock:
+    beta = weak_shock_angle(mach, theta, gamma)
+    mn1 = mach * np.sin(beta)
+    rho2 = (gamma + 1.0) * mn1 ** 2 / ((gamma - 1.0) * mn1 ** 2 + 2.0)
+    p2 = (1.0 + 2.0 * gamma / (gamma + 1.0) * (mn1 ** 2 - 1.0)) / gamma
+    un2 = mn1 / rho2  # mass flux through the shock, upstream rho = c = 1
+    ut = mach * np.cos(beta)  # tangential velocity is continuous
+    u2 = un2 * np.sin(beta) + ut * np.cos(beta)
+    v2 = -un2 * np.cos(beta) + ut * np.sin(beta)
+    return ObliqueShock(mach, theta, gamma, float(beta), (1.0, mach, 0.0, 1.0 / gamma), (rho2, u2, v2, p2))
+
+
+def oblique_shock(nx: int = 100, ny: int = 10, *, mach: float = 2.0, theta: float = np.radians(10.0),
+                  length: float = 1.0, height: float = 0.5, flux: str = "rusanov", limiter: str = "minmod",
+                  cfl: float = DEFAULT_CFL, gamma: float = 1.4, final_time: float = 2.0) -> Case:
+    """Supersonic inflow over a compression corner: 0 <= x <= length,
+    x tan(theta) <= y <= x tan(theta) + height on the sheared nx x ny mesh;
+    FixedState upstream inflow on the left, slip wall below, ExactState far
+    field on top and right; upstream state everywhere at t = 0."""
+    sol = oblique_shock_solution(mach, theta, gamma)
+    mesh = sheared_rectangle_mesh(nx, ny, (0.0, length), (0.0, height), theta)
+    far = ExactState(sol.state, sol.average)
+    bcs = {"left": FixedState.from_primitive(*sol.upstream, gamma), "bottom": SlipWall(), "top": far, "right": far}
+    solver = EulerSolver(mesh, bcs, SolverConfig(flux=flux, limiter=limiter, gamma=gamma, cfl=cfl))
+    q0 = np.broadcast_to(primitive_to_conserved(*sol.upstream, gamma), solver.state_shape).copy()
+    return Case(solver, q0, final_time)

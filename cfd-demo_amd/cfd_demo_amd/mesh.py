@@ -89,6 +89,22 @@ def rectangle_mesh(nx: int, ny: int, x_range=(0.0, 1.0), y_range=(0.0, 1.0)) -> 
     return TriangleMesh(vertices=verts, elements=elems, boundary_edges=edges)
 
 
+def sheared_rectangle_mesh(nx: int, ny: int, x_range=(0.0, 1.0), y_range=(0.0, 1.0),
+                           angle: float = 0.0) -> TriangleMesh:
+    """``rectangle_mesh`` with its vertices mapped by (x, y) -> (x, y + x
+    tan(angle)) (the generator proposed in the reference's
+    docs/adr/0012-sheared-rectangle-mesh-and-far-field-boundaries.md:20-31):
+    connectivity, the lower-left to upper-right diagonal and the tags are
+    unchanged, ``bottom`` becomes the inclined wall. A zero angle reproduces
+    ``rectangle_mesh`` exactly."""
+    if not abs(angle) < 0.5 * np.pi:
+        raise ValueError("The shear angle must lie strictly between -pi/2 and pi/2.")
+    m = rectangle_mesh(nx, ny, x_range, y_range)
+    v = np.array(m.vertices)
+    v[:, 1] = v[:, 1] + v[:, 0] * np.tan(angle)
+    return TriangleMesh(vertices=v, elements=np.array(m.elements), boundary_edges=dict(m.boundary_edges))
+
+
 @dataclass(frozen=True, eq=False)
 class MeshConnectivity:
     """neighbor_element / neighbor_face (K, 3), -1 on boundaries; sorted

@@ -9,6 +9,10 @@ Replaces the reference's per-face-node broadcast tables (``DGOperators``,
   FIXED face) | one code byte per face: bits 0-1 neighbour face, bit 2 node
   order reversed, bits 3-5 kind, bits 6-7 Cockburn-Shu partner face
 * ``lim``  (K, 8) f64: alpha[3] | beta[3] | dt_geometry | jacobian
+
+On an ``ExactState`` face the neighbour field is the first of six
+``bc_states`` rows: the exterior state at each of the face's five nodes (own
+node order), then the ghost-cell average for the limiter.
 """
 
 from __future__ import annotations
@@ -17,10 +21,12 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .boundary import FIXED_STATE, PERIODIC, SLIP_WALL, TRANSMISSIVE, Periodic
-from .mesh import MeshConnectivity, element_geometry, face_orientation
+from .boundary import EXACT_STATE, FIXED_STATE, PERIODIC, SLIP_WALL, TRANSMISSIVE, Periodic, reflect_across_faces
+from .mesh import MeshConnectivity, element_geometry, face_orientation, node_coordinates
 
-KIND_CODE = {SLIP_WALL: 1, FIXED_STATE: 2, TRANSMISSIVE: 3}
+# Boundary kind -> the kind code the kernels read (conn code bits 3-5).
+KIND_CODE = {SLIP_WALL: 1, FIXED_STATE: 2, TRANSMISSIVE: 3, EXACT_STATE: 4}
+EXACT_ROWS = 6  # bc_states rows per exact-state face: 5 face nodes + the ghost average
 _COEFF_TOL = 1e-12  # limiting.py:31
 
 
@@ -134,6 +140,37 @@ def neighbor_coefficients(conn: MeshConnectivity, geometry, shift):
     return alpha, beta, partner
 
 
+def exact_state_rows(mesh, conn: MeshConnectivity, geometry, boundary_conditions, kind, state_index, states):
+    """Append six bc_states rows per ExactState face (its five face-node
+    exterior states, evaluated once at the physical face nodes, and the ghost
+    average over the element reflected across the face) and point the face's
+    state index at the first. Returns (state_index, states)."""
+    from .reference_element import DEGREE, build_reference_triangle
+
+    ref = build_reference_triangle(DEGREE)
+    x, y = node_coordinates(mesh, ref.r, ref.s)
+    fnodes = np.asarray(ref.face_nodes).reshape(3, -1)
+    corners = mesh.vertices[mesh.elements]
+    rows = [np.asarray(states, dtype=np.float64).reshape(-1, 4)]
+    nrow = rows[0].shape[0]
+    state_index = state_index.copy()
+    for t, tag in enumerate(conn.boundary_tags):
+        bc = boundary_conditions.get(tag)
+        if bc is None or bc.kind != EXACT_STATE:
+            continue
+        e, f = np.nonzero(np.asarray(conn.face_tag) == t)
+        if e.size == 0:
+            continue
+        node = fnodes[f]  # (n, 5)
+        nodal = bc.face_states(x[node, e[:, None]], y[node, e[:, None]])  # (n, 5, 4)
+        ghost = reflect_across_faces(corners[e], geometry.normal[e], geometry.face_midpoint[e])[np.arange(e.size), f]
+        block = np.concatenate([nodal, bc.ghost_averages(ghost)[:, None, :]], axis=1)  # (n, 6, 4)
+        state_index[e, f] = nrow + EXACT_ROWS * np.arange(e.size)
+        rows.append(block.reshape(-1, 4))
+        nrow += EXACT_ROWS * e.size
+    return state_index, np.concatenate(rows)
+
+
 def time_step_geometry(geometry, degree: int = 4) -> np.ndarray:
     """min_e 2|K| / ((N+1)^2 |e|) (``time_integration.py:32-34``)."""
     return np.min(2.0 * geometry.area[:, None] / ((degree + 1) ** 2 * geometry.edge_length), axis=1)
@@ -143,6 +180,8 @@ def build_tables(mesh, conn: MeshConnectivity, geometry, boundary_conditions):
     """Assemble the compact tables. Returns (MeshTables, patched connectivity)."""
     conn, shift = periodic_patch(mesh, conn, boundary_conditions, geometry)
     kind, state_index, states = resolve_boundary_conditions(conn, boundary_conditions)
+    if np.any(kind == EXACT_STATE):
+        state_index, states = exact_state_rows(mesh, conn, geometry, boundary_conditions, kind, state_index, states)
     K = mesh.num_elements
     rev = face_orientation(mesh, conn, shift)
     alpha, beta, partner = neighbor_coefficients(conn, geometry, shift)
@@ -156,9 +195,12 @@ def build_tables(mesh, conn: MeshConnectivity, geometry, boundary_conditions):
 
     interior = conn.neighbor_element >= 0
     nbr = np.where(interior, conn.neighbor_element, np.arange(K)[:, None])
-    nbr = np.where(kind == FIXED_STATE, state_index, nbr)
+    nbr = np.where((kind == FIXED_STATE) | (kind == EXACT_STATE), state_index, nbr)
     code = (np.where(interior, conn.neighbor_face, 0) & 3) | (rev.astype(np.int64) << 2)
-    code = code | (kind.astype(np.int64) << 3)  # kind codes equal the BC kinds 0..3
+    dev_kind = np.zeros_like(kind, dtype=np.int64)
+    for k, v in KIND_CODE.items():
+        dev_kind[kind == k] = v
+    code = code | (dev_kind << 3)
     code = code | (partner << 6)
     packed = code[:, 0] | (code[:, 1] << 8) | (code[:, 2] << 16)
     cn = np.empty((K, 4), dtype=np.int32)

@@ -414,9 +414,10 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
       nb[1] = avg[1] - 2.0 * mn * nx;
       nb[2] = avg[2] - 2.0 * mn * ny;
       nb[3] = avg[3];
-    } else if (kind == BC_FIXED) {
+    } else if (kind == BC_FIXED || kind == BC_EXACT) {
+      const int64_t row = (int64_t)nbr[f] + (kind == BC_EXACT ? NFP : 0);  // an exact face's ghost average
 #pragma unroll
-      for (int c = 0; c < 4; ++c) nb[c] = a.bcs[nbr[f] * 4 + c];
+      for (int c = 0; c < 4; ++c) nb[c] = a.bcs[row * 4 + c];
     } else {
 #pragma unroll
       for (int c = 0; c < 4; ++c) nb[c] = avg[c];

@@ -150,8 +150,8 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
             qp[1] = qj[1] - 2.0 * mn * nx;
             qp[2] = qj[2] - 2.0 * mn * ny;
             qp[3] = qj[3];
-          } else if (kind == BC_FIXED) {
-            load4(a.bcs + (int64_t)nb * 4, qp);
+          } else if (kind == BC_FIXED || kind == BC_EXACT) {
+            load4(a.bcs + ((int64_t)nb + (kind == BC_EXACT ? kk : 0)) * 4, qp);
           } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) qp[c] = qj[c];

@@ -25,7 +25,10 @@ constexpr int NV = 4;        // conserved variables
 
 enum FluxId : int { FLUX_RUSANOV = 0, FLUX_ROE = 1, FLUX_HLLC = 2, FLUX_DYNAMIC = 3 };
 enum LimiterId : int { LIM_MINMOD = 0, LIM_MC = 1, LIM_VAN_LEER = 2, LIM_SUPERBEE = 3, LIM_UNLIMITED = 4 };
-enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE = 3 };
+// BC_EXACT: a prescribed exterior state per face node (rows nb + k of the
+// boundary-state table for own face node k) and the limiter's ghost average
+// at row nb + NFP.
+enum BcKind : int { BC_INTERIOR = 0, BC_SLIP = 1, BC_FIXED = 2, BC_TRANSMISSIVE = 3, BC_EXACT = 4 };
 
 constexpr double ROE_ENTROPY_FIX = 0.1;          // fluxes.py:21
 constexpr double TINY = 2.2250738585072014e-308; // finfo(float64).tiny, fluxes.py:24-25

@@ -62,8 +62,13 @@ typedef struct {
  *  conn (K x 4 int32): field f<3 is the neighbour element across face f, or the
  *       fixed-state index on a FIXED face; field 3 packs one byte per face:
  *       bits 0-1 neighbour's local face, bit 2 reversed node order,
- *       bits 3-5 kind (0 interior, 1 slip wall, 2 fixed state, 3 transmissive),
- *       bits 6-7 Cockburn-Shu partner face j of face i.
+ *       bits 3-5 kind (0 interior, 1 slip wall, 2 fixed state, 3 transmissive,
+ *       4 exact state), bits 6-7 Cockburn-Shu partner face j of face i.
+ *       On an exact-state face field f is the first of six bc_states rows:
+ *       the exterior states at own face nodes 0..4, then the ghost-cell
+ *       average the limiter uses (the far-field condition proposed in the
+ *       reference's docs/adr/0012-sheared-rectangle-mesh-and-far-field-
+ *       boundaries.md:32-44).
  *  lim  (K x 8): alpha0..2 | beta0..2 | dt_geometry | jacobian
  *  bc_states (num_bc_states x 4): conserved fixed exterior states. */
 typedef struct {

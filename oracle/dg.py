@@ -10,7 +10,8 @@ from typing import NamedTuple
 
 import numpy as np
 
-from .mesh import FIXED_STATE, PERIODIC, SLIP_WALL, TRANSMISSIVE, node_coordinates, pair_face_nodes
+from .mesh import EXACT_STATE, FIXED_STATE, PERIODIC, SLIP_WALL, TRANSMISSIVE, node_coordinates, pair_face_nodes, \
+    reflected_element
 from .physics import cartesian_fluxes
 
 
@@ -55,9 +56,30 @@ def resolve_boundary_conditions(conn, boundary_conditions):
         kind[mask] = bc.kind
         if bc.kind == FIXED_STATE:
             state[mask] = bc.state
-        elif bc.kind not in (SLIP_WALL, TRANSMISSIVE):
+        elif bc.kind not in (SLIP_WALL, TRANSMISSIVE, EXACT_STATE):
             raise ValueError(f"Unsupported boundary condition for tag {tag!r}: {bc!r}.")
     return kind, state
+
+
+def exact_boundary_states(mesh, conn, ref, kind, state, boundary_conditions):
+    """Per-face-node exterior states (K, 3, Nfp, 4) and limiter ghost
+    averages (K, 3, 4): the fixed states broadcast over their face nodes, and
+    on each ExactState face its state function at the physical face nodes and
+    its average over the element reflected across the face."""
+    nfp = ref.num_face_nodes
+    node_state = np.repeat(state[:, :, None, :], nfp, axis=2)
+    ghost = state.copy()
+    x, y = node_coordinates(mesh, ref.r, ref.s)
+    fn = np.asarray(ref.face_nodes).reshape(3, nfp)
+    corners = mesh.vertices[mesh.elements]
+    for ti, tag in enumerate(conn.boundary_tags):
+        bc = boundary_conditions[tag]
+        if bc.kind != EXACT_STATE:
+            continue
+        for e, f in zip(*np.nonzero(conn.face_tag == ti)):
+            node_state[e, f] = np.asarray(bc.state(x[fn[f], e], y[fn[f], e]), dtype=np.float64)
+            ghost[e, f] = np.asarray(bc.average(reflected_element(corners[e], f)[None]), dtype=np.float64)[0]
+    return node_state, ghost
 
 
 def build_dg_operators(mesh, conn, geo, ref, kind, state, shift=None):
@@ -88,7 +110,8 @@ def build_dg_operators(mesh, conn, geo, ref, kind, state, shift=None):
         face_scale=np.ascontiguousarray(per_face_node(fscale)),
         exterior_index=ext.reshape(3 * nfp, K),
         boundary_kind=np.ascontiguousarray(per_face_node(kind)),
-        boundary_state=np.ascontiguousarray(per_face_node(state)),
+        boundary_state=np.ascontiguousarray(per_face_node(state) if state.ndim == 3
+                                            else np.moveaxis(state, 0, 2).reshape(3 * nfp, K, 4)),
         boundary_weight=np.where(is_bnd, ew, 0.0),
         cell_weights=np.array(ref.cell_weights),
         jacobian=np.array(geo.jacobian),
@@ -106,13 +129,17 @@ def exterior_states(qm, qn, ops):
     """``dg.py:129-137``; TRANSMISSIVE (extension) takes the interior trace."""
     kind = ops.boundary_kind[..., None]
     wall = mirror_state(qm, ops.normal_x, ops.normal_y)
+    given = (kind == FIXED_STATE) | (kind == EXACT_STATE)
     return np.where(kind == SLIP_WALL, wall,
-                    np.where(kind == FIXED_STATE, ops.boundary_state,
+                    np.where(given, ops.boundary_state,
                              np.where(kind == TRANSMISSIVE, qm, qn)))
 
 
-def dg_rhs(q, ops, flux, gamma):
-    """Semi-discrete RHS and net outward boundary flux (``dg.py:140-165``)."""
+def dg_rhs(q, ops, flux, gamma, q_exterior=None):
+    """Semi-discrete RHS and net outward boundary flux (``dg.py:140-165``).
+
+    ``q_exterior`` (CPU-baseline element chunks): the whole state the
+    exterior traces are gathered from when ``q`` and ``ops`` cover a chunk."""
     nv = q.shape[-1]
     f, g = cartesian_fluxes(q, gamma)
     rx, ry, sx, sy = (row[None, :, None] for row in ops.inverse_metric)
@@ -120,7 +147,7 @@ def dg_rhs(q, ops, flux, gamma):
     divergence = np.einsum("ij,jkc->ikc", ops.differentiation, contra)
     nx, ny = ops.normal_x, ops.normal_y
     qm = q[ops.face_nodes]
-    qn = q.reshape(-1, nv)[ops.exterior_index]
+    qn = (q if q_exterior is None else q_exterior).reshape(-1, nv)[ops.exterior_index]
     qp = exterior_states(qm, qn, ops)
     fhat = flux(qm, qp, nx, ny, gamma)
     fminus = f[ops.face_nodes] * nx[..., None] + g[ops.face_nodes] * ny[..., None]

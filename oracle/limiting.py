@@ -7,7 +7,7 @@ from typing import NamedTuple
 import numpy as np
 
 from .dg import mirror_state
-from .mesh import FIXED_STATE, SLIP_WALL
+from .mesh import EXACT_STATE, FIXED_STATE, SLIP_WALL
 from .physics import pressure
 
 NEIGHBOR_FACTOR = 1.5  # limiting.py:23-24
@@ -96,18 +96,21 @@ def _minmod(a, b):
     return np.where(a * b > 0.0, np.sign(a) * np.minimum(np.abs(a), np.abs(b)), 0.0)
 
 
-def neighbor_averages(avg, ops):
+def neighbor_averages(avg, ops, avg_all=None):
     """``limiting.py:128-133``. A TRANSMISSIVE face (kind 3) keeps the own average
-    as ghost, which is what ``neighbor`` (= own element) already gives."""
-    interior = avg[ops.neighbor]
+    as ghost, which is what ``neighbor`` (= own element) already gives.
+    ``avg_all`` (CPU-baseline element chunks): the averages of every element,
+    which ``neighbor`` indexes when ``avg`` and ``ops`` cover a chunk."""
+    interior = (avg if avg_all is None else avg_all)[ops.neighbor]
     wall = mirror_state(avg[None], ops.normal_x, ops.normal_y)
     kind = ops.neighbor_kind[..., None]
-    return np.where(kind == SLIP_WALL, wall, np.where(kind == FIXED_STATE, ops.fixed_state, interior))
+    given = (kind == FIXED_STATE) | (kind == EXACT_STATE)
+    return np.where(kind == SLIP_WALL, wall, np.where(given, ops.fixed_state, interior))
 
 
-def troubled_cell_limit(q, avg, ops, phi):
+def troubled_cell_limit(q, avg, ops, phi, avg_all=None):
     """Cockburn-Shu troubled-cell limiter (``limiting.py:136-163``)."""
-    diff = neighbor_averages(avg, ops) - avg[None]
+    diff = neighbor_averages(avg, ops, avg_all) - avg[None]
     paired = np.take_along_axis(diff, ops.partner_face[..., None], axis=0)
     sn = NEIGHBOR_FACTOR * (ops.alpha[..., None] * diff + ops.beta[..., None] * paired)
     trace = q[ops.midpoint_nodes] - avg[None]
@@ -161,11 +164,11 @@ def positivity_limit(q, avg, gamma, return_theta=False):
     return (out, theta_r * np.where(theta < 1.0, theta, 1.0)) if return_theta else out
 
 
-def apply_limiters(q, ops, phi, gamma, limit_slopes, return_flags=False):
+def apply_limiters(q, ops, phi, gamma, limit_slopes, return_flags=False, avg_all=None):
     """``limiting.py:203-216``; optionally also returns the troubled flags (K,)."""
     avg = np.einsum("i,ikc->kc", ops.average_weights, q)
     if limit_slopes:
-        q, troubled = troubled_cell_limit(q, avg, ops, phi)
+        q, troubled = troubled_cell_limit(q, avg, ops, phi, avg_all)
         count = np.sum(troubled)
     else:
         troubled = np.zeros(q.shape[1], dtype=bool)

@@ -20,6 +20,10 @@ FIXED_STATE = 2  # boundary.py:14
 # boundary weight; a periodic face is patched into an interior face.
 TRANSMISSIVE = 3
 PERIODIC = 4
+# The far-field condition proposed in docs/adr/0012-sheared-rectangle-mesh-
+# and-far-field-boundaries.md:32-44: exterior states from a known solution,
+# one per face node, and the limiter ghost average over the reflected element.
+EXACT_STATE = 5
 
 
 @dataclass(frozen=True)
@@ -52,6 +56,27 @@ class Transmissive:
     """Extension: exterior state = interior trace (zero-gradient)."""
 
     kind: int = TRANSMISSIVE
+
+
+@dataclass(frozen=True, eq=False)
+class ExactState:
+    """ADR 0012 (proposed): ``state(x, y)`` -> conserved (..., 4) at the face
+    nodes; ``average(corners (n, 3, 2))`` -> (n, 4) ghost-cell averages over
+    reflected elements (required here: the oracle takes exact averages only)."""
+
+    state: object
+    average: object
+    kind: int = EXACT_STATE
+
+
+def reflected_element(corners, face):
+    """Element corners (3, 2) mirrored across the line through its face
+    ``face`` (from corner face to corner face + 1): each corner goes to twice
+    its projection on the line minus itself."""
+    a, b = corners[face], corners[(face + 1) % 3]
+    t = (b - a) / np.hypot(*(b - a))
+    proj = a + np.outer((corners - a) @ t, t)
+    return 2.0 * proj - corners
 
 
 @dataclass(frozen=True)

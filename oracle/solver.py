@@ -11,9 +11,11 @@ from typing import NamedTuple
 
 import numpy as np
 
-from .dg import build_dg_operators, cell_averages, dg_rhs, domain_integral, resolve_boundary_conditions
+from .dg import build_dg_operators, cell_averages, dg_rhs, domain_integral, exact_boundary_states, \
+    resolve_boundary_conditions
 from .limiting import apply_limiters, build_limiter_operators
-from .mesh import FixedState, SlipWall, Transmissive, build_connectivity, element_geometry, node_coordinates, \
+from .mesh import EXACT_STATE, FixedState, SlipWall, Transmissive, build_connectivity, element_geometry, \
+    node_coordinates, \
     periodic_connectivity, rectangle_mesh
 from .physics import get_interface_flux, get_limiter_function, pressure
 from .reference_element import DEGREE, build_reference_triangle
@@ -132,8 +134,13 @@ class EulerSolver:
         shift = self.periodic_shift if np.any(self.periodic_shift) else None
         self.geometry = element_geometry(mesh)
         kind, state = resolve_boundary_conditions(self.connectivity, self.boundary_conditions)
-        self.operators = build_dg_operators(mesh, self.connectivity, self.geometry, self.reference, kind, state, shift)
-        self.limiter_operators = build_limiter_operators(self.connectivity, self.geometry, self.reference, kind, state,
+        node_state, ghost = state, state
+        if np.any(kind == EXACT_STATE):
+            node_state, ghost = exact_boundary_states(mesh, self.connectivity, self.reference, kind, state,
+                                                      self.boundary_conditions)
+        self.operators = build_dg_operators(mesh, self.connectivity, self.geometry, self.reference, kind, node_state,
+                                            shift)
+        self.limiter_operators = build_limiter_operators(self.connectivity, self.geometry, self.reference, kind, ghost,
                                                          shift)
         self.dt_geometry = time_step_geometry(self.geometry, self.reference.degree)
         self.flux = get_interface_flux(config.flux)

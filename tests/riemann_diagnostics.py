@@ -179,9 +179,12 @@ def _area(tri):
     return 0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
 
 
-def _clip(poly, x, keep_left):
-    """Convex polygon (k, 2) clipped to x' <= x (keep_left) or x' >= x."""
-    d = (x - poly[:, 0]) if keep_left else (poly[:, 0] - x)
+def _clip(poly, line, keep_below):
+    """Convex polygon (k, 2) clipped to a x + b y <= c (keep_below) or >= c,
+    for line = (a, b, c)."""
+    a, b, c = line
+    d = c - (a * poly[:, 0] + b * poly[:, 1])
+    d = d if keep_below else -d
     inside = d >= 0.0
     if inside.all():
         return poly
@@ -215,27 +218,27 @@ class SplitQuadrature:
         return float(self.weight.sum())
 
 
-def split_quadrature(corners, breakpoints=(), n: int = QUAD_POINTS) -> SplitQuadrature:
-    """``corners`` (K, 3, 2) element vertices; elements that a breakpoint
-    x = b crosses are cut there and the pieces fan-triangulated."""
+def split_quadrature(corners, breakpoints=(), n: int = QUAD_POINTS, lines=()) -> SplitQuadrature:
+    """``corners`` (K, 3, 2) element vertices; elements that a vertical
+    breakpoint x = b, or a line a x + b y = c of ``lines``, crosses are cut
+    along it and the pieces fan-triangulated."""
     corners = np.asarray(corners, dtype=np.float64)
-    bps = np.sort(np.asarray(breakpoints, dtype=np.float64).ravel())
-    lo, hi = corners[..., 0].min(1), corners[..., 0].max(1)
-    slack = SPLIT_TOL * np.maximum(hi - lo, SPLIT_TOL)
-    inside = (bps[None, :] > (lo + slack)[:, None]) & (bps[None, :] < (hi - slack)[:, None])
-    cut = np.flatnonzero(inside.any(1))
+    cuts = [(1.0, 0.0, float(b)) for b in np.sort(np.asarray(breakpoints, dtype=np.float64).ravel())]
+    cuts += [tuple(float(v) for v in ln) for ln in lines]
+    size = np.maximum(np.ptp(corners[..., 0], 1), np.ptp(corners[..., 1], 1))
+    crossed = np.zeros((len(corners), len(cuts)), bool)
+    for k, (a, b, c) in enumerate(cuts):
+        d = (a * corners[..., 0] + b * corners[..., 1] - c) / np.hypot(a, b)
+        slack = SPLIT_TOL * np.maximum(size, SPLIT_TOL)
+        crossed[:, k] = (d.min(1) < -slack) & (d.max(1) > slack)
+    cut = np.flatnonzero(crossed.any(1))
     keep = np.setdiff1d(np.arange(len(corners)), cut)
     tris, parents = [corners[keep]], [keep]
     for e in cut:
-        edges = np.concatenate(([-np.inf], bps[inside[e]], [np.inf]))
-        for a, b in zip(edges[:-1], edges[1:]):
-            piece = corners[e]
-            if np.isfinite(a):
-                piece = _clip(piece, a, keep_left=False)
-            if len(piece) and np.isfinite(b):
-                piece = _clip(piece, b, keep_left=True)
-            if len(piece) < 3:
-                continue
+        pieces = [corners[e]]
+        for k in np.flatnonzero(crossed[e]):
+            pieces = [q for p in pieces for q in (_clip(p, cuts[k], True), _clip(p, cuts[k], False)) if len(q) >= 3]
+        for piece in pieces:
             fan = np.stack([np.repeat(piece[:1], len(piece) - 2, 0), piece[1:-1], piece[2:]], 1)
             tris.append(fan)
             parents.append(np.full(len(fan), e))
@@ -310,3 +313,25 @@ def shock_location(avg, face_x, window):
         raise ValueError(f"no column face lies in {window}")
     drop = np.where(inside, avg[:-1] - avg[1:], -np.inf)
     return float(face_x[int(np.argmax(drop))])
+
+
+def evaluate_at_points(solver, state, x, y, degree: int = 4):
+    """The nodal solution (Np, K, nv) at physical points (n,): each point's
+    containing element by its barycentric coordinates (the element with the
+    largest smallest coordinate, so points on faces pick one side)."""
+    m = solver.mesh
+    c = np.asarray(m.vertices)[np.asarray(m.elements)]  # (K, 3, 2)
+    j = np.stack([c[:, 1] - c[:, 0], c[:, 2] - c[:, 0]], -1)  # (K, 2, 2)
+    inv = np.linalg.inv(j)
+    pts = np.stack([np.asarray(x, float), np.asarray(y, float)], -1)  # (n, 2)
+    lam12 = np.einsum("kab,nkb->nka", inv, pts[:, None, :] - c[None, :, 0, :])  # (n, K, 2)
+    lam = np.concatenate([1.0 - lam12.sum(-1, keepdims=True), lam12], -1)
+    e = np.argmax(lam.min(-1), axis=1)
+    l12 = lam12[np.arange(len(pts)), e]
+    r, s = 2.0 * l12[:, 0] - 1.0, 2.0 * l12[:, 1] - 1.0
+    pw = monomial_powers(degree)
+    rn, sn = solver.reference.r, solver.reference.s
+    vand = rn[:, None] ** pw[None, :, 0] * sn[:, None] ** pw[None, :, 1]
+    coef = np.linalg.solve(vand, state.reshape(state.shape[0], -1)).reshape(state.shape)
+    basis = r[:, None] ** pw[:, 0] * s[:, None] ** pw[:, 1]  # (n, Np)
+    return np.einsum("nb,bnv->nv", basis, coef[:, e, :])
