@@ -12,27 +12,27 @@ roofline = the stage kernel (RHS fused with the stage combination), with
 algorithmic bytes (16 + 24 + 24)/3 = 21.33 B per DOF per launch, over its mean
 launch duration from HIP events on the solver's stream.
 cpu_baseline = the NumPy oracle (a restatement of the reference's JAX path;
-the JAX path itself is not available) on a bounded sample of the same
-workload, one core.
+the JAX path itself is not available) on all cores of this process's CPU
+affinity (at most 16), its element range split over worker processes, on a
+bounded sample of the same workload: per-step median of a few steps at the
+full 2M-element size. --cpu-plan adds the whole CPU plan: per-RHS and
+per-step medians at the Sod, vortex-plus-bumps, vortex and four-quadrant
+configurations, the full Sod run, and the one-core step as a labelled extra.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-plan]
 """
 
 from __future__ import annotations
 
+import argparse
+import ctypes as C
+import json
 import os
+import sys
+import time
+from pathlib import Path
 
-for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
-    os.environ[_v] = "1"  # the CPU baseline is a one-core NumPy run
-
-import argparse  # noqa: E402
-import ctypes as C  # noqa: E402
-import json  # noqa: E402
-import sys  # noqa: E402
-import time  # noqa: E402
-from pathlib import Path  # noqa: E402
-
-import numpy as np  # noqa: E402
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "cfd-demo_amd"))
@@ -48,42 +48,129 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, steps: int) -> dict:
-    """The oracle's SSP-RK3 step on the same workload at n x n rectangles."""
-    from oracle.mesh import Transmissive, rectangle_mesh
-    from oracle.physics import primitive_to_conserved
-    from oracle.solver import EulerSolver, SolverConfig
-    from cfd_demo_amd.cases import QUADRANT_STATES
-
-    s = EulerSolver(rectangle_mesh(n, n), {t: Transmissive() for t in ("left", "right", "bottom", "top")},
-                    SolverConfig("hllc", "mc", 1.4))
-    cx, cy = s.geometry.centroid.T
-    st = np.empty((s.mesh.num_elements, 4))
-    for key, m in (("NE", (cx > .5) & (cy > .5)), ("NW", (cx < .5) & (cy > .5)), ("SW", (cx < .5) & (cy < .5)),
-                   ("SE", (cx > .5) & (cy < .5))):
-        st[m] = primitive_to_conserved(*QUADRANT_STATES[key], 1.4)
-    q = np.broadcast_to(st[None], s.state_shape).copy()
-    # One thread: the oracle's contractions are np.einsum loops (no BLAS), and
-    # the limit keeps any BLAS call it makes on the one core reported.
-    from threadpoolctl import threadpool_limits
-
-    with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            q, rep = s.step(q, s.time_step(q))
-        el = time.perf_counter() - t0
-    dof = 60 * s.mesh.num_elements
-    model = "unknown"
+def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "unknown")
     except OSError:
-        pass
-    return {"value": 3 * dof * steps / el, "unit": "DOF-stage-updates/s", "cores": 1, "kind": "port",
-            "cpu_model": model,
-            "sample": f"oracle (NumPy restatement of the reference JAX path) {steps} SSP-RK3 steps incl. CFL dt "
-                      f"on the same four-quadrant HLLC/MC workload at {n}x{n} rectangles ({s.mesh.num_elements} "
-                      f"triangles), {el:.1f} s"}
+        return "unknown"
+
+
+def _oracle_workload(name: str, n: int):
+    """(oracle solver, state) of one benchmark configuration at size n."""
+    from oracle import mesh as om
+    from oracle.physics import primitive_to_conserved
+    from oracle.solver import EulerSolver, SolverConfig, sod_shock_tube
+    from cfd_demo_amd.cases import QUADRANT_STATES, isentropic_vortex_state, vortex_bumps_state
+
+    if name == "sod":
+        s, q, _ = sod_shock_tube(n, max(1, n // 10), flux="rusanov", limiter="minmod", cfl=0.1)
+        return s, q
+    if name == "quadrant":
+        s = EulerSolver(om.rectangle_mesh(n, n), {t: om.Transmissive() for t in ("left", "right", "bottom", "top")},
+                        SolverConfig("hllc", "mc", 1.4))
+        cx, cy = s.geometry.centroid.T
+        st = np.empty((s.mesh.num_elements, 4))
+        for key, m in (("NE", (cx > .5) & (cy > .5)), ("NW", (cx < .5) & (cy > .5)),
+                       ("SW", (cx < .5) & (cy < .5)), ("SE", (cx > .5) & (cy < .5))):
+            st[m] = primitive_to_conserved(*QUADRANT_STATES[key], 1.4)
+        return s, np.broadcast_to(st[None], s.state_shape).copy()
+    L = 10.0
+    bcs = {"left": om.Periodic("right", (-L, 0.0)), "right": om.Periodic("left", (L, 0.0)),
+           "bottom": om.Periodic("top", (0.0, -L)), "top": om.Periodic("bottom", (0.0, L))}
+    mesh = om.rectangle_mesh(n, n, (0.0, L), (0.0, L))
+    if name == "vortex":
+        s = EulerSolver(mesh, bcs, SolverConfig("roe", "unlimited", 1.4))
+        x, y = s.node_coordinates()
+        return s, isentropic_vortex_state(x, y)
+    s = EulerSolver(mesh, bcs, SolverConfig("rusanov", "unlimited", 1.4, cfl=0.1))
+    x, y = s.node_coordinates()
+    return s, vortex_bumps_state(x, y)
+
+
+def _medians(par, solver, q, rhs_reps: int, step_reps: int) -> dict:
+    """Median seconds of par.rhs(q) and par.step(q, dt) after one warm-up each."""
+    dt = solver.time_step(q)
+    out = {}
+    if rhs_reps:
+        par.rhs(q)
+        ts = []
+        for _ in range(rhs_reps):
+            t0 = time.perf_counter()
+            par.rhs(q)
+            ts.append(time.perf_counter() - t0)
+        out["rhs_median_s"] = float(np.median(ts))
+    if step_reps:
+        par.step(q, dt)
+        ts = []
+        for _ in range(step_reps):
+            t0 = time.perf_counter()
+            par.step(q, dt)
+            ts.append(time.perf_counter() - t0)
+        out["step_median_s"] = float(np.median(ts))
+    out["reps"] = [rhs_reps, step_reps]
+    return out
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    """The oracle's SSP-RK3 step of the four-quadrant workload at n x n
+    rectangles on all cores (worker processes over element chunks)."""
+    from oracle.parallel import ParallelOracle, default_workers
+
+    t0 = time.perf_counter()
+    s, q = _oracle_workload("quadrant", n)
+    setup = time.perf_counter() - t0
+    workers = default_workers()
+    with ParallelOracle(s, workers) as par:
+        med = _medians(par, s, q, 0, steps)
+    dof = 60 * s.mesh.num_elements
+    return {"value": 3 * dof / med["step_median_s"], "unit": "DOF-stage-updates/s", "cores": workers,
+            "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"oracle (NumPy restatement of the reference JAX path) SSP-RK3 step on the four-quadrant "
+                      f"HLLC/MC workload at {n}x{n} rectangles ({s.mesh.num_elements} triangles), element range "
+                      f"split over {workers} worker processes (bit-identical to the serial oracle); median of "
+                      f"{steps} steps after one warm-up step: {med['step_median_s']:.2f} s per step "
+                      f"(oracle setup {setup:.0f} s, not timed)"}
+
+
+def cpu_plan() -> dict:
+    """Per-RHS and per-step medians of the oracle on all cores at the Sod
+    (100x10), vortex-plus-bumps (200^2), vortex (512^2) and four-quadrant
+    (1000^2) configurations (five timed calls each after one warm-up), the
+    full Sod run to t = 0.2 at CFL 0.1, and the one-core step of the
+    vortex-plus-bumps field as a labelled extra."""
+    from oracle.parallel import ParallelOracle, default_workers
+
+    workers = default_workers()
+    out = {"cores": workers, "cpu_model": _cpu_model(), "kind": "port", "configs": {}}
+    for name, n in (("sod", 100), ("bumps", 200), ("vortex", 512), ("quadrant", 1000)):
+        s, q = _oracle_workload(name, n)
+        with ParallelOracle(s, workers if s.mesh.num_elements > 20000 else min(workers, 4)) as par:
+            med = _medians(par, s, q, 5, 5)
+            med["workers"] = par.workers
+            if name == "sod":
+                t0 = time.perf_counter()
+                qq, t, steps = q, 0.0, 0
+                while t < 0.2:
+                    dt = min(s.time_step(qq), 0.2 - t)
+                    qq, _ = par.step(qq, dt)
+                    t = 0.2 if dt == 0.2 - t else t + dt
+                    steps += 1
+                med["full_run_s"] = time.perf_counter() - t0
+                med["full_run_steps"] = steps
+        K = s.mesh.num_elements
+        med["elements"] = K
+        med["dof_stage_per_s"] = 3 * 60 * K / med["step_median_s"]
+        out["configs"][name] = med
+        log(f"cpu plan {name} {K}: {med}")
+        if name == "bumps":
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(limits=1):
+                t0 = time.perf_counter()
+                s.step(q, s.time_step(q))
+                out["one_core_bumps_step_s"] = time.perf_counter() - t0
+    return out
 
 
 def batched_bench(n_cases: int, steps: int, seed: int = 0) -> dict:
@@ -252,9 +339,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1000, help="rectangles per side (K = 2 n^2)")
-    ap.add_argument("--cpu-n", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-n", type=int, default=1000, help="rectangles per side of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed oracle steps of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-plan", action="store_true",
+                    help="also time the whole CPU plan (about 5 minutes): medians at four configurations")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-cases", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=30)
@@ -268,7 +357,14 @@ def main() -> int:
     ap.add_argument("--adjoint-fd", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--cpu-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_only:
+        out = {"cpu_baseline": cpu_baseline(args.cpu_n, args.cpu_steps)}
+        if args.cpu_plan:
+            out["cpu_plan"] = cpu_plan()
+        print(json.dumps(out), flush=True)
+        return 0
     if args.copy_probe:
         import torch
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -405,8 +501,18 @@ def main() -> int:
         line["cases"] = cases_bench(args.c3_n)
     if not args.no_cpu:
         log(f"cpu baseline: oracle at {args.cpu_n}x{args.cpu_n}, {args.cpu_steps} steps")
-        line["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_steps)
-        log(f"cpu baseline {line['cpu_baseline']['value']:.4g} DOF-stage/s")
+        # A child process that has not opened the GPU runs the oracle and
+        # forks its workers, so none of them holds the device.
+        cmd = [sys.executable, "-u", str(Path(__file__).resolve()), "--cpu-only", "--cpu-n", str(args.cpu_n),
+               "--cpu-steps", str(args.cpu_steps)] + (["--cpu-plan"] if args.cpu_plan else [])
+        import subprocess
+
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, check=True, close_fds=True)
+        cpu = json.loads(res.stdout.decode().strip().splitlines()[-1])
+        line["cpu_baseline"] = cpu["cpu_baseline"]
+        if "cpu_plan" in cpu:
+            line["cpu_plan"] = cpu["cpu_plan"]
+        log(f"cpu baseline {line['cpu_baseline']['value']:.4g} DOF-stage/s on {line['cpu_baseline']['cores']} cores")
     print(json.dumps(line), flush=True)
     solver.close()
     return 0
