@@ -67,6 +67,18 @@ class CfdDeviceMesh(C.Structure):
 
 CFD_STATIC_MAX_ELEMENTS = 65536
 
+
+class CfdMeshInput(C.Structure):
+    _fields_ = [("num_vertices", C.c_int64), ("vertices", _dp), ("num_elements", C.c_int64), ("elements", _i64p),
+                ("num_tags", C.c_int32), ("tag_names", C.c_char_p), ("tag_offsets", _i64p), ("tag_edges", _i64p),
+                ("tag_kind", _ip), ("tag_state", _ip), ("tag_partner", _ip), ("tag_shift", _dp)]
+
+
+class CfdMeshOutput(C.Structure):
+    _fields_ = [("geo", _dp), ("conn", _ip), ("lim", _dp), ("neighbor_element", _i64p), ("neighbor_face", _i64p),
+                ("face_tag", _i64p), ("centroid", _dp), ("normal", _dp), ("edge_length", _dp),
+                ("face_midpoint", _dp), ("periodic_shift", _dp)]
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "cfd_last_error": (C.c_char_p, []),
@@ -105,6 +117,7 @@ SIGNATURES = {
     "cfd_adjoint_stats": (C.c_int, [C.c_void_p, C.POINTER(CfdAdjointStats)]),
     "cfd_flux_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double, _dp]),
     "cfd_limiter_pointwise": (C.c_int, [C.c_int, C.c_int64, _dp, _dp]),
+    "cfd_build_tables": (C.c_int, [C.POINTER(CfdMeshInput), C.POINTER(CfdMeshOutput)]),
 }
 
 _lib = None

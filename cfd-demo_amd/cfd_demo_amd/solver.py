@@ -19,10 +19,10 @@ from typing import NamedTuple
 import numpy as np
 
 from . import _lib
-from .mesh import TriangleMesh, build_connectivity, element_geometry, node_coordinates
+from .mesh import TriangleMesh, node_coordinates
 from .physics import NUM_VARIABLES, get_interface_flux, get_limiter_function
 from .reference_element import DEGREE, build_reference_triangle
-from .tables import build_tables, time_step_geometry
+from .tables import assemble
 
 DEFAULT_CFL = 0.5  # time_integration.py:18
 MAX_REJECTIONS = 10  # time_integration.py:19
@@ -103,10 +103,9 @@ class EulerSolver:
         self.config = config
         self.boundary_conditions = MappingProxyType(dict(boundary_conditions))
         self.reference = build_reference_triangle(DEGREE)
-        self.geometry = element_geometry(mesh)
-        self.tables, self.connectivity = build_tables(mesh, build_connectivity(mesh), self.geometry,
-                                                      self.boundary_conditions)
-        self.dt_geometry = time_step_geometry(self.geometry, self.reference.degree)
+        # Tables, patched connectivity and geometry from the native assembler.
+        self.tables, self.connectivity, self.geometry = assemble(mesh, self.boundary_conditions)
+        self.dt_geometry = self.tables.lim[:, 6].copy()  # time_step_geometry (time_integration.py:32-34)
         self.flux = get_interface_flux(config.flux)
         self.phi = get_limiter_function(config.limiter)
         self.limiter_operators = LimiterTables(self)

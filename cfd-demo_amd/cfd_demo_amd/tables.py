@@ -17,17 +17,17 @@ node order), then the ghost-cell average for the limiter.
 
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
 
 from .boundary import EXACT_STATE, FIXED_STATE, PERIODIC, SLIP_WALL, TRANSMISSIVE, Periodic, reflect_across_faces
-from .mesh import MeshConnectivity, element_geometry, face_orientation, node_coordinates
+from .mesh import ElementGeometry, MeshConnectivity, node_coordinates
 
 # Boundary kind -> the kind code the kernels read (conn code bits 3-5).
 KIND_CODE = {SLIP_WALL: 1, FIXED_STATE: 2, TRANSMISSIVE: 3, EXACT_STATE: 4}
 EXACT_ROWS = 6  # bc_states rows per exact-state face: 5 face nodes + the ghost average
-_COEFF_TOL = 1e-12  # limiting.py:31
 
 
 @dataclass(frozen=True, eq=False)
@@ -71,75 +71,6 @@ def resolve_boundary_conditions(conn: MeshConnectivity, boundary_conditions):
     return kind, state_index, np.asarray(states, dtype=np.float64).reshape(-1, 4)
 
 
-def periodic_patch(mesh, conn: MeshConnectivity, boundary_conditions, geometry):
-    """Turn periodic tag pairs into interior faces: returns (connectivity,
-    shift (K, 3, 2)) where shift translates the neighbour onto this element."""
-    ne = np.array(conn.neighbor_element)
-    nf = np.array(conn.neighbor_face)
-    ft = np.array(conn.face_tag)
-    shift = np.zeros(ne.shape + (2,))
-    tags = conn.boundary_tags
-    for t, tag in enumerate(tags):
-        bc = boundary_conditions.get(tag)
-        if not isinstance(bc, Periodic):
-            continue
-        if bc.partner not in tags or not isinstance(boundary_conditions.get(bc.partner), Periodic):
-            raise ValueError(f"Periodic tag {tag!r} needs a periodic partner tag {bc.partner!r}.")
-        mine = np.argwhere(np.asarray(conn.face_tag) == t)
-        theirs = np.argwhere(np.asarray(conn.face_tag) == tags.index(bc.partner))
-        if len(mine) != len(theirs):
-            raise ValueError(f"Periodic tags {tag!r} and {bc.partner!r} do not match.")
-        key_m = geometry.face_midpoint[mine[:, 0], mine[:, 1]]
-        key_t = geometry.face_midpoint[theirs[:, 0], theirs[:, 1]] + np.asarray(bc.shift, dtype=np.float64)
-        scale = geometry.edge_length[mine[:, 0], mine[:, 1]]
-        # Match by lexicographic order of the rounded midpoints.
-        h = float(np.min(scale))
-        om = np.lexsort(np.round(key_m / (1e-3 * h)).T[::-1])
-        ot = np.lexsort(np.round(key_t / (1e-3 * h)).T[::-1])
-        if np.any(np.linalg.norm(key_m[om] - key_t[ot], axis=-1) > 1e-9 * scale[om]):
-            raise ValueError(f"Periodic tags {tag!r} and {bc.partner!r} do not match.")
-        e, f = mine[om, 0], mine[om, 1]
-        ne[e, f] = theirs[ot, 0]
-        nf[e, f] = theirs[ot, 1]
-        shift[e, f] = np.asarray(bc.shift, dtype=np.float64)
-        ft[e, f] = -1
-    return MeshConnectivity(ne, nf, conn.boundary_tags, ft), shift
-
-
-def neighbor_coefficients(conn: MeshConnectivity, geometry, shift):
-    """Cockburn-Shu alpha, beta >= 0 and partner face for every face: m_i - b0
-    = alpha (b_i - b0) + beta (b_j - b0), trying j = i+1 then i+2; boundary
-    ghosts reflect b0 across the face (``limiting.py:51-94``; same arithmetic,
-    so the coefficients agree bit for bit)."""
-    cen = geometry.centroid
-    nb = conn.neighbor_element
-    inner = cen[np.maximum(nb, 0)] + shift
-    off = np.sum((geometry.face_midpoint - cen[:, None, :]) * geometry.normal, axis=-1, keepdims=True)
-    ghost = cen[:, None, :] + 2.0 * off * geometry.normal
-    d = np.where((nb >= 0)[..., None], inner, ghost) - cen[:, None, :]
-    tgt = geometry.face_midpoint - cen[:, None, :]
-    K = cen.shape[0]
-    alpha = np.zeros((K, 3))
-    beta = np.zeros((K, 3))
-    partner = -np.ones((K, 3), dtype=np.int64)
-    for face in range(3):
-        for step in (1, 2):
-            other = (face + step) % 3
-            di, dj, t = d[:, face], d[:, other], tgt[:, face]
-            det = di[:, 0] * dj[:, 1] - di[:, 1] * dj[:, 0]
-            safe = np.where(det == 0.0, 1.0, det)
-            a = (t[:, 0] * dj[:, 1] - t[:, 1] * dj[:, 0]) / safe
-            b = (di[:, 0] * t[:, 1] - di[:, 1] * t[:, 0]) / safe
-            take = (det != 0.0) & (a >= -_COEFF_TOL) & (b >= -_COEFF_TOL) & (partner[:, face] < 0)
-            alpha[take, face] = np.maximum(a[take], 0.0)
-            beta[take, face] = np.maximum(b[take], 0.0)
-            partner[take, face] = other
-    if np.any(partner < 0):
-        el, fc = np.argwhere(partner < 0)[0]
-        raise ValueError(f"No nonnegative Cockburn-Shu neighbor coefficients for face {fc} of element {el}.")
-    return alpha, beta, partner
-
-
 def exact_state_rows(mesh, conn: MeshConnectivity, geometry, boundary_conditions, kind, state_index, states):
     """Append six bc_states rows per ExactState face (its five face-node
     exterior states, evaluated once at the physical face nodes, and the ghost
@@ -176,42 +107,82 @@ def time_step_geometry(geometry, degree: int = 4) -> np.ndarray:
     return np.min(2.0 * geometry.area[:, None] / ((degree + 1) ** 2 * geometry.edge_length), axis=1)
 
 
-def build_tables(mesh, conn: MeshConnectivity, geometry, boundary_conditions):
-    """Assemble the compact tables. Returns (MeshTables, patched connectivity)."""
-    conn, shift = periodic_patch(mesh, conn, boundary_conditions, geometry)
-    kind, state_index, states = resolve_boundary_conditions(conn, boundary_conditions)
-    if np.any(kind == EXACT_STATE):
-        state_index, states = exact_state_rows(mesh, conn, geometry, boundary_conditions, kind, state_index, states)
+def assemble(mesh, boundary_conditions):
+    """The compact tables, the connectivity (periodic faces patched to
+    interior faces) and the geometry of ``mesh`` under ``boundary_conditions``
+    from the native assembler (cfd_build_tables; host code, no GPU). Returns
+    (MeshTables, MeshConnectivity, ElementGeometry). Boundary-condition checks
+    and messages are those of ``resolve_boundary_conditions`` (``dg.py:52-73``)."""
+    from . import _lib
+
+    tags = tuple(sorted(mesh.boundary_edges))
+    missing = sorted(set(tags) - set(boundary_conditions))
+    if missing:
+        raise ValueError(f"Boundary tags without a boundary condition: {missing}.")
+    unknown = sorted(set(boundary_conditions) - set(tags))
+    if unknown:
+        raise ValueError(f"Boundary conditions for tags that are not in the mesh: {unknown}.")
+    T = len(tags)
+    kind = np.zeros(T, dtype=np.int32)
+    state = np.zeros(T, dtype=np.int32)
+    partner = -np.ones(T, dtype=np.int32)
+    shift = np.zeros((T, 2))
+    states = []
+    for t, tag in enumerate(tags):
+        bc = boundary_conditions[tag]
+        if bc.kind == PERIODIC:
+            if bc.partner not in tags or not isinstance(boundary_conditions.get(bc.partner), Periodic):
+                raise ValueError(f"Periodic tag {tag!r} needs a periodic partner tag {bc.partner!r}.")
+            partner[t] = tags.index(bc.partner)
+            shift[t] = np.asarray(bc.shift, dtype=np.float64)
+        elif bc.kind not in KIND_CODE:
+            raise ValueError(f"Unsupported boundary condition for tag {tag!r}: {bc!r}.")
+        kind[t] = bc.kind
+        if bc.kind == FIXED_STATE:
+            state[t] = len(states)
+            states.append(bc.state)
     K = mesh.num_elements
-    rev = face_orientation(mesh, conn, shift)
-    alpha, beta, partner = neighbor_coefficients(conn, geometry, shift)
-
+    verts = np.ascontiguousarray(mesh.vertices, dtype=np.float64)
+    elems = np.ascontiguousarray(mesh.elements, dtype=np.int64)
+    edges = [np.ascontiguousarray(mesh.boundary_edges[t], dtype=np.int64).reshape(-1, 2) for t in tags]
+    offsets = np.zeros(T + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum([len(e) for e in edges])
+    tag_edges = np.ascontiguousarray(np.concatenate(edges) if edges else np.zeros((0, 2), np.int64))
+    names = "\n".join(tags).encode()
     geo = np.empty((K, 16))
-    geo[:, 0], geo[:, 1], geo[:, 2], geo[:, 3] = geometry.rx, geometry.ry, geometry.sx, geometry.sy
-    geo[:, 4:7] = geometry.normal[..., 0]
-    geo[:, 7:10] = geometry.normal[..., 1]
-    geo[:, 10:13] = 0.5 * geometry.edge_length / geometry.jacobian[:, None]  # dg.py:99
-    geo[:, 13:16] = 0.5 * geometry.edge_length  # dg.py:101
-
-    interior = conn.neighbor_element >= 0
-    nbr = np.where(interior, conn.neighbor_element, np.arange(K)[:, None])
-    nbr = np.where((kind == FIXED_STATE) | (kind == EXACT_STATE), state_index, nbr)
-    code = (np.where(interior, conn.neighbor_face, 0) & 3) | (rev.astype(np.int64) << 2)
-    dev_kind = np.zeros_like(kind, dtype=np.int64)
-    for k, v in KIND_CODE.items():
-        dev_kind[kind == k] = v
-    code = code | (dev_kind << 3)
-    code = code | (partner << 6)
-    packed = code[:, 0] | (code[:, 1] << 8) | (code[:, 2] << 16)
     cn = np.empty((K, 4), dtype=np.int32)
-    cn[:, :3] = nbr
-    cn[:, 3] = packed.astype(np.int32)
-
     lim = np.empty((K, 8))
-    lim[:, 0:3] = alpha
-    lim[:, 3:6] = beta
-    lim[:, 6] = time_step_geometry(geometry)
-    lim[:, 7] = geometry.jacobian
-    tables = MeshTables(geo=np.ascontiguousarray(geo), conn=cn, lim=lim,
-                        bc_states=np.ascontiguousarray(states), boundary_kind=kind, periodic_shift=shift)
-    return tables, conn
+    ne = np.empty((K, 3), dtype=np.int64)
+    nf = np.empty((K, 3), dtype=np.int64)
+    ftag = np.empty((K, 3), dtype=np.int64)
+    cen = np.empty((K, 2))
+    nrm = np.empty((K, 3, 2))
+    length = np.empty((K, 3))
+    midp = np.empty((K, 3, 2))
+    pshift = np.empty((K, 3, 2))
+    i64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    inp = _lib.CfdMeshInput(len(verts), dp(verts), K, i64(elems), T, names, i64(offsets), i64(tag_edges),
+                            i32(kind), i32(state), i32(partner), dp(shift))
+    out = _lib.CfdMeshOutput(dp(geo), i32(cn), dp(lim), i64(ne), i64(nf), i64(ftag), dp(cen), dp(nrm), dp(length),
+                             dp(midp), dp(pshift))
+    _lib.check(_lib.load().cfd_build_tables(C.byref(inp), C.byref(out)))
+    def ro(a):  # freshly written arrays (or views of them): freeze in place, no copy
+        a.setflags(write=False)
+        return a
+
+    conn = MeshConnectivity(ro(ne), ro(nf), tags, ro(ftag))
+    geometry = ElementGeometry(jacobian=ro(lim[:, 7].copy()), rx=ro(geo[:, 0]), ry=ro(geo[:, 1]), sx=ro(geo[:, 2]),
+                               sy=ro(geo[:, 3]), centroid=ro(cen), normal=ro(nrm), edge_length=ro(length),
+                               face_midpoint=ro(midp))
+    face_kind = np.where(ftag >= 0, kind[np.maximum(ftag, 0)], 0).astype(np.int32)
+    states = np.asarray(states, dtype=np.float64).reshape(-1, 4)
+    if np.any(face_kind == EXACT_STATE):
+        index = np.asarray(cn[:, :3], dtype=np.int32)
+        index, states = exact_state_rows(mesh, conn, geometry, boundary_conditions, face_kind, index, states)
+        exact = face_kind == EXACT_STATE
+        cn[:, :3] = np.where(exact, index, cn[:, :3])
+    tables = MeshTables(geo=geo, conn=cn, lim=lim, bc_states=np.ascontiguousarray(states), boundary_kind=face_kind,
+                        periodic_shift=pshift)
+    return tables, conn, geometry

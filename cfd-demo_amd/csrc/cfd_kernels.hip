@@ -43,7 +43,8 @@ __constant__ double c_linb[NP * 3];
 // ---------------------------------------------------------------------------
 template <int FLUX, int MODE, int SB>
 __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_stage(StageArgs a) {
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  int blk = xcd_block(blockIdx.x, gridDim.x);
+  if constexpr (FLUX == FLUX_DYNAMIC) blk = listed_block(a.bv, blk, LBLOCK / SB);
   const int64_t K = a.K;
   const int64_t e0 = (int64_t)blk * SB;
   const int ne = (int)((K - e0) < SB ? (K - e0) : SB);
@@ -256,7 +257,7 @@ __device__ __forceinline__ void fold_stage_rows(const LimitArgs& a, int blk, dou
 }
 
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int blk = listed_block(a.bv, xcd_block(blockIdx.x, gridDim.x), 1);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
   part_neutral(row);
@@ -514,7 +515,7 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
   __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
   part_neutral(row);
-  const int b = blockIdx.x;
+  const int b = listed_block(a.bv, blockIdx.x, 1);
   const int n = a.bv.wcount[b];
   if (n > 0) {
     const int cs = a.bv.lcase[b];
@@ -708,3 +709,4 @@ __global__ void k_limiter_pointwise(int64_t n, int id, const double* __restrict_
 #include "cfd_capi.inc"
 #include "cfd_batched.inc"
 #include "cfd_adjoint.inc"
+#include "cfd_tables.inc"

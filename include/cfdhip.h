@@ -288,6 +288,51 @@ int cfd_adjoint_forward_replay(CfdAdjoint* adj, const double* q0_dev, double* qn
 int cfd_adjoint_time_steps(CfdAdjoint* adj, double* dts);
 int cfd_adjoint_stats(CfdAdjoint* adj, CfdAdjointStats* stats);
 
+/* Host-side table assembly (no GPU needed): connectivity and boundary tags,
+ * the periodic patch, face-node orientation, affine geometry, Cockburn-Shu
+ * coefficients and CFL geometry of a straight-sided triangle mesh, written as
+ * the compact tables CfdMeshTables describes. Replaces the reference's NumPy
+ * setup in build_connectivity (mesh.py:135-185), element_geometry
+ * (mesh.py:216-243), pair_face_nodes (mesh.py:254-291), build_dg_operators
+ * (dg.py:76-116) and neighbor_coefficients (limiting.py:51-94), with the same
+ * error texts (CFD_EINVAL). Boundary tags are given in sorted name order,
+ * tag_names newline-separated; tag kinds 1 slip wall, 2 fixed state (tag_state
+ * = its bc_states row), 3 transmissive, 4 periodic (tag_partner, tag_shift:
+ * x_here = x_partner + shift), 5 exact state (the field of such a face is
+ * left 0 for the caller to point at its bc_states rows). The output arrays
+ * geo (K x 16), conn (K x 4), lim (K x 8) are required; the others may be
+ * NULL: neighbor_element, neighbor_face, face_tag (K x 3 int64, periodic
+ * faces patched to interior faces), centroid (K x 2), normal (K x 3 x 2),
+ * edge_length (K x 3), face_midpoint (K x 3 x 2), periodic_shift (K x 3 x 2). */
+typedef struct {
+  int64_t num_vertices;
+  const double* vertices;     /* Nv x 2 */
+  int64_t num_elements;
+  const int64_t* elements;    /* K x 3, counterclockwise */
+  int32_t num_tags;
+  const char* tag_names;
+  const int64_t* tag_offsets; /* num_tags + 1: tag t owns edges tag_offsets[t] .. tag_offsets[t + 1] - 1 */
+  const int64_t* tag_edges;   /* vertex pairs */
+  const int32_t* tag_kind;
+  const int32_t* tag_state;
+  const int32_t* tag_partner;
+  const double* tag_shift;    /* num_tags x 2 */
+} CfdMeshInput;
+typedef struct {
+  double* geo;
+  int32_t* conn;
+  double* lim;
+  int64_t* neighbor_element;
+  int64_t* neighbor_face;
+  int64_t* face_tag;
+  double* centroid;
+  double* normal;
+  double* edge_length;
+  double* face_midpoint;
+  double* periodic_shift;
+} CfdMeshOutput;
+int cfd_build_tables(const CfdMeshInput* in, CfdMeshOutput* out);
+
 /* Pointwise interface flux (fluxes.py:28-143) on n face states: qm, qp (n,4),
  * nx, ny (n), out (n,4); host arrays. */
 int cfd_flux_pointwise(int flux_id, int64_t n, const double* qm, const double* qp, const double* nx,
