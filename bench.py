@@ -211,6 +211,80 @@ def batched_bench(n_cases: int, steps: int, seed: int = 0) -> dict:
             "speedup_vs_loop": (work / t_batch) / (work_loop / t_loop)}
 
 
+def setup_bench() -> dict:
+    """Host setup time of the native table assembly (cfd_build_tables through
+    tables.assemble: connectivity, periodic patch, orientation, geometry,
+    Cockburn-Shu coefficients) at 2M and 8M triangles, second call (the
+    first pays the OS's first touch of the freshly allocated arrays)."""
+    from cfd_demo_amd.boundary import Transmissive
+    from cfd_demo_amd.mesh import rectangle_mesh
+    from cfd_demo_amd.tables import assemble
+
+    out = {}
+    bcs = {t: Transmissive() for t in ("left", "right", "bottom", "top")}
+    for n in (1000, 2000):
+        t0 = time.perf_counter()
+        mesh = rectangle_mesh(n, n)
+        t_mesh = time.perf_counter() - t0
+        ts = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            assemble(mesh, bcs)
+            ts.append(time.perf_counter() - t0)
+        out[f"{2 * n * n}"] = {"rectangle_mesh_s": t_mesh, "assemble_first_s": ts[0], "assemble_s": ts[1]}
+        del mesh
+    log(f"native table assembly: {out}")
+    return out
+
+
+def batched_full_bench(n_cases: int, seed: int, step_cap: int) -> dict:
+    """The batched workload as specified: every seeded tube run to its own
+    final time, as one batch (hipGraph replay of one launch per flux per
+    stage) and as the plain loop of EulerSolver.run over the cases. A tube
+    whose CFL step collapses stops at ``step_cap`` steps on both sides (the
+    reference's loop has no cap and would not end)."""
+    from cfd_demo_amd.batched import BatchedSolver, random_shock_tube_specs, shock_tube_case
+
+    cases = [shock_tube_case(s) for s in random_shock_tube_specs(n_cases, seed=seed)]
+    batch = BatchedSolver(cases)
+    batch.begin()
+    sb, _ = batch.autotune(reps=3)
+    batch.advance(2)  # warm-up (graph capture)
+    t0 = time.perf_counter()
+    res_b = batch.run(max_steps=step_cap)
+    t_batch = time.perf_counter() - t0
+    log(f"batched full runs: {t_batch:.1f}s")
+    batch.close()
+    for c in cases:
+        c.solver._context()
+    res_l = []
+    t0 = time.perf_counter()
+    for i, c in enumerate(cases):
+        res_l.append(c.solver.run(c.initial_state, c.final_time, max_steps=step_cap))
+        if i % 32 == 31:
+            log(f"per-case loop: {i + 1} cases in {time.perf_counter() - t0:.1f}s")
+    t_loop = time.perf_counter() - t0
+    for c in cases:
+        c.solver.close()
+    steps_b = np.array([r.steps for r in res_b])
+    steps_l = np.array([r.steps for r in res_l])
+    done_b = np.array([r.time == c.final_time for r, c in zip(res_b, cases)])
+    done_l = np.array([r.time == c.final_time for r, c in zip(res_l, cases)])
+    work_b = sum(3 * 60 * c.solver.mesh.num_elements * int(n) for c, n in zip(cases, steps_b))
+    work_l = sum(3 * 60 * c.solver.mesh.num_elements * int(n) for c, n in zip(cases, steps_l))
+    diff = max(float(np.abs(a.state[..., 0] - b.state[..., 0]).max()) for a, b in zip(res_b, res_l))
+    log(f"batched full runs: {n_cases} tubes, batch {t_batch:.2f}s, loop {t_loop:.2f}s, speedup "
+        f"{t_loop / t_batch:.2f}, done {int(done_b.sum())}/{int(done_l.sum())}, steps {steps_b.min()}-{steps_b.max()}, "
+        f"max |d rho| batch vs loop {diff:.2e}")
+    return {"cases": n_cases, "seed": seed, "step_cap": step_cap, "stage_block": sb,
+            "batch_wall_s": t_batch, "loop_wall_s": t_loop, "speedup_vs_loop": t_loop / t_batch,
+            "batched_dof_stage_per_s": work_b / t_batch, "loop_dof_stage_per_s": work_l / t_loop,
+            "cases_at_final_time": [int(done_b.sum()), int(done_l.sum())],
+            "capped_cases": [int(i) for i in np.flatnonzero(~done_b)],
+            "steps_min_max_total": [int(steps_b.min()), int(steps_b.max()), int(steps_b.sum())],
+            "same_steps": bool(np.array_equal(steps_b, steps_l)), "max_density_diff_batch_vs_loop": diff}
+
+
 def adjoint_bench(n: int, batch: int, steps: int, checkpoint_every: int, fd_dirs: int, seed: int = 0) -> dict:
     """Vortex plus 8 seeded Gaussian bumps (field b uses seed + b), periodic
     [0,10]^2 on n x n rectangles, Rusanov, unlimited, CFL 0.1; J = sum over
@@ -347,6 +421,9 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-cases", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--batched-full", action="store_true",
+                    help="run every batched tube to its own final time, batch and per-case loop")
+    ap.add_argument("--batched-cap", type=int, default=40000, help="step cap of a tube whose CFL step collapses")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cases", action="store_true", help="skip the full c1 (Sod) and c3 (vortex) runs")
     ap.add_argument("--c3-n", type=int, default=512)
@@ -494,6 +571,9 @@ def main() -> int:
     }
     if not args.no_batched:
         line["batched"] = batched_bench(args.batched_cases, args.batched_steps)
+    line["setup"] = setup_bench()
+    if args.batched_full:
+        line["batched_full"] = batched_full_bench(args.batched_cases, 0, args.batched_cap)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_bench(args.adjoint_n, args.adjoint_batch, args.adjoint_steps,
                                         args.adjoint_checkpoint, args.adjoint_fd)

@@ -93,7 +93,8 @@ class BatchedSolver:
             c = np.empty((npad, 4), dtype=np.int32)
             cc = t.conn.astype(np.int64)
             kind = np.stack([(cc[:, 3] >> (8 * f + 3)) & 7 for f in range(3)], axis=1)
-            nbr = np.where(kind == 2, cc[:, :3] + nstates, cc[:, :3] + first)
+            given = (kind == 2) | (kind == 4)  # fixed-state and exact-state faces index bc_states
+            nbr = np.where(given, cc[:, :3] + nstates, cc[:, :3] + first)
             c[:K, :3] = nbr
             c[:K, 3] = t.conn[:, 3]
             pad_ids = first + np.arange(K, npad)

@@ -85,7 +85,17 @@ struct BatchView {
   const int* limiter;      // limiter id per case
   const double* dt;        // dt of the current attempt per case (<= 0: case inactive)
   int* wcount;             // per 256-element block: worklist length
+  // The 256-element blocks a launch covers (nullptr: all of them): workgroup
+  // w of a launch works on the w-th listed block (or the part of it its
+  // workgroup size covers), so launches shrink as cases finish.
+  const int* sblk = nullptr;
 };
+
+// Element block of workgroup w when each 256-element block holds `per`
+// workgroups: through the launch's block list, or w itself.
+__device__ __forceinline__ int listed_block(const BatchView& bv, int w, int per) {
+  return bv.sblk ? bv.sblk[w / per] * per + w % per : w;
+}
 
 struct StageArgs {
   const double* q;
