@@ -57,7 +57,7 @@ class CfdAdjointStats(C.Structure):
     _fields_ = [("num_steps", C.c_int64), ("checkpoints", C.c_int64), ("tape_records", C.c_int64),
                 ("tape_peak", C.c_int64), ("tape_regrows", C.c_int64), ("positivity_scaled", C.c_int64),
                 ("forward_ms", C.c_double), ("backward_ms", C.c_double), ("recompute_ms", C.c_double),
-                ("device_bytes", C.c_int64)]
+                ("device_bytes", C.c_int64), ("skipped_passes", C.c_int64)]
 
 
 class CfdDeviceMesh(C.Structure):

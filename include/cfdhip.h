@@ -274,6 +274,8 @@ typedef struct {
   double backward_ms;        /* last backward, host wall clock */
   double recompute_ms;       /* part of backward_ms spent recomputing segments */
   int64_t device_bytes;      /* device memory held by the run */
+  int64_t skipped_passes;    /* limiter passes the last backward's recomputation skipped (unlimited runs:
+                                passes the forward found to repair nothing) */
 } CfdAdjointStats;
 
 int cfd_adjoint_create(CfdCtx* ctx, int64_t num_steps, const double* dts, int checkpoint_every, CfdAdjoint** out);

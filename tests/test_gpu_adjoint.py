@@ -112,6 +112,39 @@ def test_march_adjoint_two_launch_limiter_matches_autograd(flux, bcs, monkeypatc
     assert rel_err(qn.cpu().numpy(), qn_oracle) <= 1e-12
     got = run.backward(torch.from_numpy(lam).cuda()).cpu().numpy()
     assert _vrel(got, want) <= 1e-10, (flux, bcs, _vrel(got, want))
+    st = run.stats()
+    assert st["skipped_passes"] > 0  # the smooth march repairs nothing: recomputation skips its passes
+    run.close()
+
+
+def test_march_adjoint_positivity_repair_in_a_recomputed_segment(monkeypatch):
+    """An unlimited march from a state with one low-pressure element (1 % of
+    the smooth field's pressure): the positivity limiter repairs it in passes
+    0 and 2-5 of the first segment, which the backward recomputes, while pass
+    1 repairs nothing and is skipped there. The gradient still matches the
+    autograd oracle."""
+    from oracle.physics import conserved_to_primitive, primitive_to_conserved
+
+    torch = _torch()
+    monkeypatch.setenv("CFDHIP_LIMIT_ONEPASS", "0")
+    g, o = _pair("rusanov", "unlimited", "periodic", n=4, seed=18)
+    x, y = o.node_coordinates()
+    rho, u, v, p = conserved_to_primitive(smooth_state(x, y, amp=0.3), 1.4)
+    p = p.copy()
+    p[:, 5] *= 0.01
+    q0 = np.ascontiguousarray(primitive_to_conserved(rho, u, v, p, 1.4))
+    dts = np.full(5, 0.2 * o.time_step(q0))
+    lam = np.random.default_rng(19).standard_normal(q0.shape)
+    want, qn_oracle = at.march_vjp(o, q0, dts, lam)
+    run = adj.DiscreteAdjoint(g, len(dts), dts=dts, checkpoint_every=2)
+    qn = run.forward(torch.from_numpy(q0).cuda())
+    assert rel_err(qn.cpu().numpy(), qn_oracle) <= 1e-12
+    got = run.backward(torch.from_numpy(lam).cuda()).cpu().numpy()
+    st = run.stats()
+    print(f"positivity-scaled records {st['positivity_scaled']}, skipped passes {st['skipped_passes']}, "
+          f"rel err {_vrel(got, want):.2e}", flush=True)
+    assert st["positivity_scaled"] > 0 and st["skipped_passes"] > 0
+    assert _vrel(got, want) <= 1e-10
     run.close()
 
 
