@@ -298,3 +298,47 @@ def test_pointwise_flux_branches_match_oracle(flux, stage_layout):
     assert np.all(np.isfinite(want)) and np.all(np.isfinite(got))
     scale = np.maximum(np.abs(want).max(axis=1, keepdims=True), 1e-300)
     assert np.all(np.abs(got - want) <= 1e-13 * scale)
+
+
+def _kinetic_node_state(g):
+    """Uniform state with one node whose kinetic energy is ~1e10 times its
+    pressure, and whose correctly rounded pressure (pressure(), no
+    contraction) lies within 1e-18 below the positivity floor 1e-13."""
+    from oracle.physics import pressure
+
+    q = np.broadcast_to(np.array([1.0, 0.0, 0.0, 2.5]), g.state_shape).copy()
+    rho, mom = 0.5, 0.03  # ke = 0.5 * mom^2 / rho = 9e-4
+    E = 0.5 * mom * mom / rho + 0.999999e-13 / 0.4
+    node = np.array([rho, mom, 0.0, E])
+    for _ in range(10000):
+        p = float(pressure(node, 1.4))
+        if 1e-13 - 1e-18 < p < 1e-13:
+            break
+        node[3] = np.nextafter(node[3], -np.inf if p >= 1e-13 else np.inf)
+    assert 1e-13 - 1e-18 < float(pressure(node, 1.4)) < 1e-13
+    q[7, 5] = node
+    return q
+
+
+def test_positivity_filter_sends_kinetic_dominated_node():
+    """The summary's pre-filter uses the refined hardware reciprocal; a node
+    whose correctly rounded pressure is just below 1e-13 while kinetic
+    energy dominates must still reach the exact repair: the limiter and a
+    dt = 0 step (stage-kernel summary) both match the oracle, which repairs it."""
+    from oracle.physics import pressure
+
+    g, o = _pair("fixed", limiter="unlimited", nx=4, ny=4)
+    q = _kinetic_node_state(g)
+    print(f"case kinetic-dominated node p={float(pressure(q[7, 5], 1.4)):.17e}", flush=True)
+    q_ref = o.limit(q)
+    repair = np.abs(q_ref[:, 5] - q[:, 5]).max()
+    assert repair > 0.0  # the oracle repairs the element (by about 1e-16)
+    q_gpu = g.limit(q)
+    assert rel_err(q_gpu, q_ref) <= 1e-12
+    assert np.abs(q_gpu[:, 5] - q_ref[:, 5]).max() <= 1e-3 * repair  # repaired as the oracle did
+    assert float(pressure(q_gpu[7, 5], 1.4)) >= 1e-13 * (1.0 - 1e-9)
+    s_gpu, rep = g.step(q, 0.0)
+    s_ref, rref = o.step(q, 0.0)
+    assert rel_err(s_gpu, s_ref) <= 1e-11
+    assert float(rref.min_pressure) >= 1e-13 * (1.0 - 1e-9)
+    assert rep.min_pressure == pytest.approx(float(rref.min_pressure), rel=1e-10)
