@@ -1,0 +1,70 @@
+"""The batched workload run as specified: all 256 seeded shock tubes (seed 0,
+Nx ~ U{50..400}, fluxes and limiters cycling) batched, each to its own final
+time, with a 20,000-step cap (the reference's loop has none and would not end
+for the tubes whose CFL step collapses).
+
+* every tube conserves mass, momentum and energy against its boundary-flux
+  integral to 1e-10 of the conserved totals;
+* every tube that reaches its final time with all its waves still inside
+  [0, 1] (transmissive ends are not the unbounded problem once a wave leaves)
+  is compared with the exact Riemann solution by quadrature split at the
+  wave edges: density L1 at most 0.1 for each and 0.01 at the median (the
+  error is resolution-bound: a contact and a shock a few cells apart on the
+  coarser tubes give the largest);
+* each capped tube is accounted for: either its nodal density or pressure
+  came within round-off of the positivity floor 1e-13 (at most 1e-12: the
+  positivity scaling lands a few ulps above the floor; a p = 4 near-vacuum state whose wave
+  speed makes the CFL step collapse, as the oracle marches too), or it is
+  within 20 % of its final time with positive minima (a fine tube whose
+  strong waves need more than 20,000 steps).
+"""
+
+import numpy as np
+import pytest
+
+from cfd_demo_amd.batched import BatchedSolver, random_shock_tube_specs, shock_tube_case
+from riemann_diagnostics import RiemannSolution, riemann_l1
+
+pytestmark = pytest.mark.gpu
+
+CAP = 20000
+
+
+def _integral(s, q):
+    w = s.reference.cell_weights[:, None] * s.geometry.jacobian[None, :]
+    return np.einsum("ik,ikc->c", w, q)
+
+
+def test_all_seeded_tubes_to_their_final_times():
+    specs = random_shock_tube_specs(256, seed=0)
+    cases = [shock_tube_case(s) for s in specs]
+    print(f"case 256 batched tubes to final time, cap {CAP}", flush=True)
+    batch = BatchedSolver(cases)
+    res = batch.run(max_steps=CAP)
+    batch.close()
+    l1s, capped, skipped = [], [], 0
+    for i, (sp, c, r) in enumerate(zip(specs, cases, res)):
+        s = c.solver
+        total = _integral(s, np.abs(c.initial_state))
+        defect = _integral(s, r.state) - _integral(s, c.initial_state) + r.boundary_flux_integral
+        assert np.all(np.abs(defect) <= 1e-10 * np.maximum(total, 1.0)), (i, defect)
+        if r.time != c.final_time:
+            assert r.steps + r.rejected_steps == CAP  # every batched step is an attempt
+            capped.append(i)
+            floor = min(r.min_density, r.min_pressure) <= 1e-12
+            slow = r.time >= 0.8 * c.final_time and r.min_density > 0 and r.min_pressure > 0
+            assert floor or slow, (i, r.time, c.final_time, r.min_density, r.min_pressure)
+            continue
+        L, R = sp["left"], sp["right"]
+        sol = RiemannSolution((L[0], L[1], L[3]), (R[0], R[1], R[3]), sp["x0"])
+        waves = sol.wave_positions(r.time).values()
+        if min(waves) < 0.0 or max(waves) > 1.0:
+            skipped += 1
+            continue
+        l1s.append(riemann_l1(s, r.state, sol, r.time, n=4)["density"])
+    l1s = np.array(l1s)
+    print(f"{len(l1s)} tubes compared with their exact solutions: density L1 median {np.median(l1s):.2e}, "
+          f"max {l1s.max():.2e}; {skipped} with a wave outside [0, 1]; capped {capped}", flush=True)
+    assert len(l1s) >= 150
+    assert l1s.max() <= 0.1 and np.median(l1s) <= 0.01
+    assert len(capped) <= 40
