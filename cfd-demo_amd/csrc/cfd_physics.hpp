@@ -69,6 +69,7 @@ template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(const ad::Dual<N>& 
 template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(double a, const ad::Dual<N>& b) { return a / b; }
 template <int N> __device__ __forceinline__ ad::Dual<N> fdiv(const ad::Dual<N>& a, double b) { return a / b; }
 template <int N> __device__ __forceinline__ ad::Dual<N> fsqrt(const ad::Dual<N>& a) { return sqrt(a); }
+template <int N> __device__ __forceinline__ ad::Dual<N> frcp(const ad::Dual<N>& a) { return 1.0 / a; }
 
 template <class T>
 struct PrimT {
@@ -82,7 +83,7 @@ template <class T>
 __device__ __forceinline__ PrimT<T> to_prim(const T q[4], double gm1) {
   PrimT<T> w;
   w.rho = q[0];
-  w.inv = fdiv(1.0, q[0]);
+  w.inv = frcp(q[0]);
   w.u = q[1] * w.inv;
   w.v = q[2] * w.inv;
   w.p = gm1 * (q[3] - 0.5 * (q[1] * w.u + q[2] * w.v));
@@ -228,13 +229,16 @@ __device__ __forceinline__ void flux_hllc(const SideT<T>& m, const SideT<T>& p, 
   const SideT<T>& k = left ? m : p;
   const T s_k = left ? s_m : s_p;
   const T mass_k = left ? mass_m : mass_p;
-  const T factor = fdiv(mass_k, safe_den(s_k - s_star));
+  // factor = mass_k / (S_k - S*); factor * p_k / mass_k = p_k / (S_k - S*),
+  // so one reciprocal serves both terms of the star energy.
+  const T rden = frcp(safe_den(s_k - s_star));
+  const T factor = mass_k * rden;
   const T shift = s_star - k.un;
   T star[4];
   star[0] = factor;
   star[1] = factor * (k.w.u + shift * nx);
   star[2] = factor * (k.w.v + shift * ny);
-  star[3] = factor * (k.q[3] * k.w.inv + shift * (s_star + fdiv(k.w.p, safe_den(mass_k))));
+  star[3] = factor * (k.q[3] * k.w.inv + shift * s_star) + shift * (k.w.p * rden);
 #pragma unroll
   for (int c = 0; c < 4; ++c) out[c] = k.f[c] + s_k * (star[c] - k.q[c]);
 }
