@@ -342,6 +342,10 @@ def adjoint_bench(n: int, batch: int, steps: int, checkpoint_every: int, fd_dirs
             "fd_directions": fd_dirs, "fd_max_rel_err": worst, "device_gib": st["device_bytes"] / 2 ** 30}
 
 
+TRAFFIC_SOURCE = ("not measured in this run: HBM bytes per launch read from profiles/stage_kernel_c2_traffic.json "
+                  "(rocprofv3 FETCH_SIZE and WRITE_SIZE on this workload, scaled by a 1 GiB copy kernel)")
+
+
 def traffic_from_profile(kernel_prefix: str):
     p = ROOT / "profiles" / "stage_kernel_c2_traffic.json"
     if not p.exists():
@@ -555,13 +559,15 @@ def main() -> int:
             "unit": "GB/s",
             "frac": stage_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": TRAFFIC_SOURCE,
             "kernel": "k_stage4<HLLC> (RHS + SSP-RK3 combine + limiter summary, four lanes per element), mean of the 3 stage launches",
             "algorithmic_bytes_per_launch": STAGE_BYTES_PER_DOF * dof,
             "mean_launch_ms": stage_ms,
         },
         "rhs_roofline": {
             "bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS,
-            "kernel": "k_stage4<HLLC, RHS only>", "traffic": rhs_traffic, "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
+            "kernel": "k_stage4<HLLC, RHS only>", "traffic": rhs_traffic, "traffic_source": TRAFFIC_SOURCE,
+            "algorithmic_bytes_per_launch": RHS_BYTES_PER_DOF * dof,
             "mean_launch_ms": rhs_ms.value,
             "median_launch_ms": rhs_med,
             "median_launch_ms_perturbed_input": rhs_med_pt,
