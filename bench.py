@@ -92,6 +92,7 @@ def _medians(par, solver, q, rhs_reps: int, step_reps: int) -> dict:
     """Median seconds of par.rhs(q) and par.step(q, dt) after one warm-up each."""
     dt = solver.time_step(q)
     out = {}
+    K = solver.mesh.num_elements
     if rhs_reps:
         par.rhs(q)
         ts = []
@@ -99,6 +100,7 @@ def _medians(par, solver, q, rhs_reps: int, step_reps: int) -> dict:
             t0 = time.perf_counter()
             par.rhs(q)
             ts.append(time.perf_counter() - t0)
+            log(f"cpu {K} elements: oracle rhs {ts[-1]:.3f} s")
         out["rhs_median_s"] = float(np.median(ts))
     if step_reps:
         par.step(q, dt)
@@ -107,6 +109,7 @@ def _medians(par, solver, q, rhs_reps: int, step_reps: int) -> dict:
             t0 = time.perf_counter()
             par.step(q, dt)
             ts.append(time.perf_counter() - t0)
+            log(f"cpu {K} elements: oracle step {ts[-1]:.3f} s")
         out["step_median_s"] = float(np.median(ts))
     out["reps"] = [rhs_reps, step_reps]
     return out
@@ -156,6 +159,8 @@ def cpu_plan() -> dict:
                     qq, _ = par.step(qq, dt)
                     t = 0.2 if dt == 0.2 - t else t + dt
                     steps += 1
+                    if steps % 2000 == 0:
+                        log(f"cpu sod full run: {steps} steps, t = {t:.4f}, {time.perf_counter() - t0:.0f} s")
                 med["full_run_s"] = time.perf_counter() - t0
                 med["full_run_steps"] = steps
         K = s.mesh.num_elements
