@@ -13,8 +13,9 @@ their closed forms in fluxes.py:
   Numerical Methods for Fluid Dynamics, 3rd ed., eqs. 10.37-10.39) in the
   face frame (normal, tangential), rotated back, with the reference's
   Einfeldt/Roe wave-speed estimates; both star branches with a contact speed
-  S* different from the side's normal velocity, all four components. The two
-  star states have equal pressure and normal velocity S*.
+  S* different from the side's normal velocity, all four components. Both
+  star states move with normal velocity S* and satisfy the integral
+  consistency condition of the HLLC fan (Toro eq. 10.29).
 """
 
 import numpy as np
