@@ -22,9 +22,10 @@
 // a fixed flux; k_stage below (one lane per element, 60 accumulators, two
 // waves per SIMD) serves the run-time flux of batched solves, the
 // summary-only mode and other workgroup sizes. Contractions on FP64 MFMA
-// (in two lane layouts) and an LDS-staged one-lane-per-element variant were
-// measured slower on MI355X, whose FP64 matrix and vector rates are equal,
-// and are not kept.
+// (in two lane layouts; MI355X's FP64 matrix and vector rates are equal), an
+// LDS-staged one-lane-per-element variant and a sixteen-lanes-per-element
+// layout with DPP row-broadcast FMAs were all measured slower and are not
+// kept.
 #include "cfd_device.hpp"
 
 namespace cfd {
