@@ -16,7 +16,11 @@ constexpr int LIM_STRIDE = 8;    // doubles of limiter geometry per element in t
 constexpr int LIM_DEV = 6;       // ... on the device: alpha[3], beta[3] (dt geometry and J go to their own arrays)
 constexpr double INF = __builtin_huge_val();
 
-// Summary layout (per element, written by k_stage, read by k_limit).
+// Summary layout (per element, written by the stage kernels, read by the
+// limiter): seven double2 pieces stored piece-major, piece p of element e at
+// ((double2*)summ)[p * K + e], so a wave's 16-byte loads of one piece for
+// consecutive elements are contiguous. Offsets below are in doubles within
+// the element's gathered row (piece p = doubles 2p, 2p + 1).
 constexpr int SUMM_AVG = 0;    // 4: cell average of the unlimited stage state
 constexpr int SUMM_MID = 4;    // 6: (rho, E) at the three face-midpoint nodes
 constexpr int SUMM_RMIN = 10;  // min nodal density
@@ -341,9 +345,26 @@ __device__ __forceinline__ double pressure_floor(double p, double E, double ke, 
   return p - gm1 * (fabs(E) + fabs(ke)) * 0x1p-47;
 }
 
+__device__ __forceinline__ double2* summ_piece(double* summ, int64_t K, int p, int64_t e) {
+  return reinterpret_cast<double2*>(summ) + (int64_t)p * K + e;
+}
+
+// Gathers pieces [0, npieces) of element e's summary into sm.
+template <int NPIECES = SUMM_STRIDE / 2>
+__device__ __forceinline__ void load_summary(const double* summ, int64_t K, int64_t e, double* sm) {
+  const double2* s = reinterpret_cast<const double2*>(summ);
+#pragma unroll
+  for (int p = 0; p < NPIECES; ++p) {
+    const double2 v = s[(int64_t)p * K + e];
+    sm[2 * p] = v.x;
+    sm[2 * p + 1] = v.y;
+  }
+}
+
 // Per-element limiter summary of a stage state held in registers.
 template <bool WITH_LAMBDA>
-__device__ __forceinline__ void write_summary(const double q[NP][NV], double gamma, double* sm) {
+__device__ __forceinline__ void write_summary(const double q[NP][NV], double gamma, double* summ, int64_t K,
+                                              int64_t e) {
   const double gm1 = gamma - 1.0;
   double avg[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -375,14 +396,13 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
     for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
   }
   const int flags = ((rmin < eps) ? SF_NEED_RHO : 0) | (need_p ? SF_NEED_P : 0) | (finite ? 0 : SF_NONFINITE);
-  double2* o = reinterpret_cast<double2*>(sm);
-  o[0] = make_double2(avg[0], avg[1]);
-  o[1] = make_double2(avg[2], avg[3]);
-  o[2] = make_double2(q[MID_NODE[0]][0], q[MID_NODE[0]][3]);
-  o[3] = make_double2(q[MID_NODE[1]][0], q[MID_NODE[1]][3]);
-  o[4] = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
-  o[5] = make_double2(rmin, pmin);
-  o[6] = make_double2((double)flags, lam);
+  *summ_piece(summ, K, 0, e) = make_double2(avg[0], avg[1]);
+  *summ_piece(summ, K, 1, e) = make_double2(avg[2], avg[3]);
+  *summ_piece(summ, K, 2, e) = make_double2(q[MID_NODE[0]][0], q[MID_NODE[0]][3]);
+  *summ_piece(summ, K, 3, e) = make_double2(q[MID_NODE[1]][0], q[MID_NODE[1]][3]);
+  *summ_piece(summ, K, 4, e) = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
+  *summ_piece(summ, K, 5, e) = make_double2(rmin, pmin);
+  *summ_piece(summ, K, 6, e) = make_double2((double)flags, lam);
 }
 
 // _minmod (limiting.py:124-125).
@@ -416,7 +436,7 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
     pf[f] = (fc >> 6) & 3;
     double nb[4];
     if (kind == BC_INTERIOR) {
-      load4(a.summ + (int64_t)nbr[f] * SUMM_STRIDE, nb);
+      load_summary<2>(a.summ, a.K, nbr[f], nb);
     } else if (kind == BC_SLIP) {
       const double nx = geo[4 + f], ny = geo[7 + f];
       const double mn = avg[1] * nx + avg[2] * ny;

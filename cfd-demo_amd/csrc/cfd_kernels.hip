@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
   }
 
   if constexpr (MODE != MODE_RHS) {
-    if (live) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ + e * SUMM_STRIDE);
+    if (live) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ, K, e);
   }
   if constexpr (MODE != MODE_SUMMARY) block_reduce_bflux<SB>(bfl, a.part + (int64_t)blk * PART_STRIDE);
 }
@@ -270,24 +270,13 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
     a.limiter = a.bv.limiter[cs];
     a.limit_slopes = a.limiter != LIM_UNLIMITED;
   }
-  // The block's summary rows through LDS: contiguous 16-byte loads across the
-  // block instead of one 112-byte row per lane (rows padded to 18 doubles so
-  // the per-row reads below are free of bank conflicts).
-  __shared__ double srow[LBLOCK][18];
-  {
-    const int64_t r0 = (int64_t)blk * LBLOCK;
-    const int nrow = (int)min((int64_t)LBLOCK, a.K - r0);
-    const double2* src = reinterpret_cast<const double2*>(a.summ + r0 * SUMM_STRIDE);
-    for (int i = threadIdx.x; i < nrow * (SUMM_STRIDE / 2); i += LBLOCK) {
-      const double2 v = src[i];
-      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2))] = v.x;
-      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2)) + 1] = v.y;
-    }
-    __syncthreads();
-  }
+  // The summary pieces are stored piece-major: each of the seven 16-byte
+  // loads is contiguous across the block's lanes.
   if (live) {
+    double sm[SUMM_STRIDE];
+    load_summary(a.summ, a.K, e, sm);
     int entry = 0;
-    if (detect_element(a, e, &srow[threadIdx.x][0], row, entry)) {
+    if (detect_element(a, e, sm, row, entry)) {
       if (batched) {
         const int slot = atomicAdd(&a.bv.wcount[blk], 1);
         a.work[(int64_t)blk * LBLOCK + slot] = entry;
@@ -310,8 +299,10 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect_direct(LimitArgs a) {
   double row[PART_COLS];
   part_neutral(row);
   if (e < a.K) {
+    double sm[SUMM_STRIDE];
+    load_summary(a.summ, a.K, e, sm);
     int entry = 0;
-    if (detect_element(a, e, a.summ + e * SUMM_STRIDE, row, entry)) {
+    if (detect_element(a, e, sm, row, entry)) {
       const int slot = atomicAdd(a.work_count, 1);
       a.work[slot] = entry;
     }
@@ -368,7 +359,8 @@ __device__ __forceinline__ void repair_group(const LimitArgs& a, int entry, int6
   const bool troubled = entry < 0;
   const int64_t K = a.K;
   const int j = min(lane, NP - 1);
-  const double* sm = a.summ + e * SUMM_STRIDE;
+  double sm[10];  // pieces 0-4: cell average, midpoint (rho, E)
+  load_summary<5>(a.summ, K, e, sm);
   double avg[4];
   load4(sm, avg);
   double q[4];
@@ -548,11 +540,12 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_block(LimitArgs a) {
   {
     const int64_t r0 = (int64_t)blk * LBLOCK;
     const int nrow = (int)min((int64_t)LBLOCK, a.K - r0);
-    const double2* src = reinterpret_cast<const double2*>(a.summ + r0 * SUMM_STRIDE);
+    const double2* src = reinterpret_cast<const double2*>(a.summ);
     for (int i = threadIdx.x; i < nrow * (SUMM_STRIDE / 2); i += LBLOCK) {
-      const double2 v = src[i];
-      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2))] = v.x;
-      srow[i / (SUMM_STRIDE / 2)][2 * (i % (SUMM_STRIDE / 2)) + 1] = v.y;
+      const int p = i / nrow, r = i - p * nrow;
+      const double2 v = src[(int64_t)p * a.K + r0 + r];
+      srow[r][2 * p] = v.x;
+      srow[r][2 * p + 1] = v.y;
     }
     __syncthreads();
   }
