@@ -362,6 +362,41 @@ def traffic_from_profile(kernel_prefix: str):
         return None
 
 
+def c2_full_bench(n: int, segments: int = 6) -> dict:
+    """The four-quadrant workload run to its final time t = 0.3 through
+    ``EulerSolver.run``, as ``segments`` successive calls over 0.3/segments
+    each (the equations are autonomous; each call clamps its last step to its
+    end time), so every call returns within seconds and reports progress:
+    steps, wall time, DOF-stage updates/s, the conservation defect against
+    the boundary-flux integral, the minima and the troubled maximum."""
+    from cfd_demo_amd.cases import four_quadrant
+
+    case = four_quadrant(n)
+    s, q0, T = case.solver, case.initial_state, case.final_time
+    s.run(q0, 1e-6)  # context and tables on the device
+    q, steps, wall, bflux = q0, 0, 0.0, np.zeros(4)
+    rmin, pmin, troubled = np.inf, np.inf, 0
+    for k in range(segments):
+        t0 = time.perf_counter()
+        r = s.run(q, T / segments)
+        wall += time.perf_counter() - t0
+        q, steps = r.state, steps + r.steps
+        bflux += r.boundary_flux_integral
+        rmin, pmin, troubled = min(rmin, r.min_density), min(pmin, r.min_pressure), max(troubled, r.max_troubled_cells)
+        log(f"c2 full run: segment {k + 1}/{segments}, {steps} steps, {wall:.1f}s")
+    K = s.mesh.num_elements
+    defect = s.integrate(q) - s.integrate(q0) + bflux
+    scale = s.integrate(np.abs(q0))
+    out = {"elements": K, "final_time": T, "segments": segments, "steps": steps, "wall_s": wall,
+           "ms_per_step": 1e3 * wall / steps, "dof_stage_per_s": 3.0 * 60 * K * steps / wall,
+           "conservation_defect_rel": float(np.max(np.abs(defect) / scale)), "min_density": rmin,
+           "min_pressure": pmin, "max_troubled_cells": troubled}
+    log(f"c2 full run to t = {T}: {steps} steps in {wall:.1f}s ({out['ms_per_step']:.3f} ms/step), "
+        f"defect {out['conservation_defect_rel']:.1e}, min rho {rmin:.3e}, min p {pmin:.3e}")
+    s.close()
+    return out
+
+
 def cases_bench(c3_n: int) -> dict:
     """Full runs through ``EulerSolver.run``: the reference's Sod case (c1:
     100x10, Rusanov, minmod, CFL 0.1, t = 0.2) and the isentropic vortex (c3:
@@ -430,6 +465,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-cases", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=30)
+    ap.add_argument("--c2-full", action="store_true",
+                    help="also run the four-quadrant workload to its final time t = 0.3 (about 48,000 steps)")
     ap.add_argument("--batched-full", action="store_true",
                     help="run every batched tube to its own final time, batch and per-case loop")
     ap.add_argument("--batched-cap", type=int, default=40000, help="step cap of a tube whose CFL step collapses")
@@ -583,6 +620,8 @@ def main() -> int:
     if not args.no_batched:
         line["batched"] = batched_bench(args.batched_cases, args.batched_steps)
     line["setup"] = setup_bench()
+    if args.c2_full:
+        line["c2_full_run"] = c2_full_bench(args.n)
     if args.batched_full:
         line["batched_full"] = batched_full_bench(args.batched_cases, 0, args.batched_cap)
     if not args.no_adjoint:

@@ -504,7 +504,11 @@ __device__ __forceinline__ void repair_group(const LimitArgs& a, int entry, int6
   }
 }
 
-__global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
+// Held to four waves per SIMD (128 VGPRs, from 164): the repair is a chain of
+// dependent loads per element, so more groups in flight pay; late in the
+// four-quadrant run (a quarter of the elements repaired per pass) the limiter
+// went from 0.495 to 0.415 ms per stage. Five waves spilled and were slower.
+__global__ void __launch_bounds__(FBLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_limit_fix(LimitArgs a) {
   __shared__ double qs[FBLOCK / RG][NP][4];
   __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
@@ -526,7 +530,7 @@ __global__ void __launch_bounds__(FBLOCK) k_limit_fix(LimitArgs a) {
 
 // Batched variant: block b repairs the worklist of 256-element block b, so
 // each case's report rows stay contiguous; it re-arms its list counter.
-__global__ void __launch_bounds__(FBLOCK) k_limit_fix_blocks(LimitArgs a) {
+__global__ void __launch_bounds__(FBLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_limit_fix_blocks(LimitArgs a) {
   __shared__ double qs[FBLOCK / RG][NP][4];
   __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
