@@ -215,6 +215,29 @@ __device__ __forceinline__ void load_geo(const double* geo, int64_t e, ElemGeo& 
   }
 }
 
+// Face frame of an affine triangle from its inverse-Jacobian entries. The
+// reference faces s = -1, r + s = 0 and r = -1 map to the physical edge
+// vectors 2J (sy, -sx), 2J (-ry - sy, rx + sx) and 2J (ry, -rx), J the volume
+// Jacobian 1 / (rx sy - ry sx). With g_f = -(sx, sy), (rx + sx, ry + sy),
+// -(rx, ry) and sigma the sign of J: outward unit normal sigma g_f / |g_f|,
+// face scale 0.5 |e_f| / J = sigma |g_f|, half edge length 0.5 |e_f| = |g_f| |J|.
+// The geometry row holds the same quantities computed from the vertices
+// (mesh.py:216-243, dg.py:99-101); the two agree to a few ulps. The
+// right-hand-side kernel computes them instead of loading them: nine of the
+// row's sixteen doubles, loaded by every lane of an element, are a third of
+// the data its loads return.
+__device__ __forceinline__ void face_frame(double rx, double ry, double sx, double sy, int f, double& nx, double& ny,
+                                           double& fs) {
+  const double gx = f == 0 ? -sx : (f == 1 ? rx + sx : -rx);
+  const double gy = f == 0 ? -sy : (f == 1 ? ry + sy : -ry);
+  const bool ccw = rx * sy - ry * sx > 0.0;
+  const double len = fsqrt(gx * gx + gy * gy);
+  const double il = frcp(len);
+  nx = gx * (ccw ? il : -il);
+  ny = gy * (ccw ? il : -il);
+  fs = ccw ? len : -len;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous range of element blocks so neighbour traces hit its own L2.
 __device__ __forceinline__ int xcd_block(int b, int nb) {
