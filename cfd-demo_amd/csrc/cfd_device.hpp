@@ -20,7 +20,10 @@ constexpr double INF = __builtin_huge_val();
 // limiter): seven double2 pieces stored piece-major, piece p of element e at
 // ((double2*)summ)[p * K + e], so a wave's 16-byte loads of one piece for
 // consecutive elements are contiguous. Offsets below are in doubles within
-// the element's gathered row (piece p = doubles 2p, 2p + 1).
+// the element's gathered row (piece p = doubles 2p, 2p + 1), except that the
+// cell average is stored as the pieces (rho, E) and (rho u, rho v), so the
+// troubled-cell test, which compares densities and energies only, gathers one
+// piece per neighbour; load_summary puts the average back in order.
 constexpr int SUMM_AVG = 0;    // 4: cell average of the unlimited stage state
 constexpr int SUMM_MID = 4;    // 6: (rho, E) at the three face-midpoint nodes
 constexpr int SUMM_RMIN = 10;  // min nodal density
@@ -372,15 +375,21 @@ __device__ __forceinline__ double2* summ_piece(double* summ, int64_t K, int p, i
   return reinterpret_cast<double2*>(summ) + (int64_t)p * K + e;
 }
 
-// Gathers pieces [0, npieces) of element e's summary into sm.
+// Where the two doubles of stored piece p go in the gathered row: the cell
+// average's pieces hold (avg 0, avg 3) and (avg 1, avg 2).
+__device__ __forceinline__ constexpr int summ_lo(int p) { return p == 0 ? 0 : (p == 1 ? 1 : 2 * p); }
+__device__ __forceinline__ constexpr int summ_hi(int p) { return p == 0 ? 3 : (p == 1 ? 2 : 2 * p + 1); }
+
+// Gathers pieces [0, npieces) of element e's summary into sm (the row order
+// of the SUMM_ offsets).
 template <int NPIECES = SUMM_STRIDE / 2>
 __device__ __forceinline__ void load_summary(const double* summ, int64_t K, int64_t e, double* sm) {
   const double2* s = reinterpret_cast<const double2*>(summ);
 #pragma unroll
   for (int p = 0; p < NPIECES; ++p) {
     const double2 v = s[(int64_t)p * K + e];
-    sm[2 * p] = v.x;
-    sm[2 * p + 1] = v.y;
+    sm[summ_lo(p)] = v.x;
+    sm[summ_hi(p)] = v.y;
   }
 }
 
@@ -419,8 +428,8 @@ __device__ __forceinline__ void write_summary(const double q[NP][NV], double gam
     for (int c = 0; c < 4; ++c) finite = finite && isfinite(q[i][c]);
   }
   const int flags = ((rmin < eps) ? SF_NEED_RHO : 0) | (need_p ? SF_NEED_P : 0) | (finite ? 0 : SF_NONFINITE);
-  *summ_piece(summ, K, 0, e) = make_double2(avg[0], avg[1]);
-  *summ_piece(summ, K, 1, e) = make_double2(avg[2], avg[3]);
+  *summ_piece(summ, K, 0, e) = make_double2(avg[0], avg[3]);
+  *summ_piece(summ, K, 1, e) = make_double2(avg[1], avg[2]);
   *summ_piece(summ, K, 2, e) = make_double2(q[MID_NODE[0]][0], q[MID_NODE[0]][3]);
   *summ_piece(summ, K, 3, e) = make_double2(q[MID_NODE[1]][0], q[MID_NODE[1]][3]);
   *summ_piece(summ, K, 4, e) = make_double2(q[MID_NODE[2]][0], q[MID_NODE[2]][3]);
@@ -438,8 +447,12 @@ __device__ __forceinline__ double minmod2(double a, double b) {
   return 0.0;
 }
 
-// Troubled-cell detection (limiting.py:128-151). Also returns the scaled
-// neighbour deviations nu * (alpha d_i + beta d_j) for the reconstruction.
+// Troubled-cell detection (limiting.py:128-151). With SLOPES it also returns
+// the scaled neighbour deviations nu * (alpha d_i + beta d_j) of all four
+// components for the reconstruction; without, only the density and energy
+// entries of sn are meaningful (the test reads no others), and a neighbour's
+// average costs one 16-byte piece instead of two.
+template <bool SLOPES>
 __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, const double avg[4],
                                                 const double* sm, double sn[3][4]) {
 #pragma clang fp contract(off)
@@ -459,7 +472,15 @@ __device__ __forceinline__ bool detect_troubled(const LimitArgs& a, int64_t e, c
     pf[f] = (fc >> 6) & 3;
     double nb[4];
     if (kind == BC_INTERIOR) {
-      load_summary<2>(a.summ, a.K, nbr[f], nb);
+      if constexpr (SLOPES) {
+        load_summary<2>(a.summ, a.K, nbr[f], nb);
+      } else {
+        const double2 v = reinterpret_cast<const double2*>(a.summ)[nbr[f]];  // piece 0: (rho, E)
+        nb[0] = v.x;
+        nb[1] = avg[1];
+        nb[2] = avg[2];
+        nb[3] = v.y;
+      }
     } else if (kind == BC_SLIP) {
       const double nx = geo[4 + f], ny = geo[7 + f];
       const double mn = avg[1] * nx + avg[2] * ny;

@@ -224,7 +224,7 @@ __device__ __forceinline__ bool detect_element(const LimitArgs& a, int64_t e, co
   bool troubled = false;
   if (a.limit_slopes) {
     double sn[3][4];
-    troubled = detect_troubled(a, e, avg, sm, sn);
+    troubled = detect_troubled<false>(a, e, avg, sm, sn);
     if (a.force) troubled = a.force[e] != 0;
   }
   const bool work = troubled || (fl & (SF_NEED_RHO | SF_NEED_P));
@@ -389,7 +389,7 @@ __device__ __forceinline__ void repair_group(const LimitArgs& a, int entry, int6
   double q[4];
   load4(a.q + ((int64_t)j * K + e) * 4, q);
   double sn[3][4];
-  if (troubled) detect_troubled(a, e, avg, sm, sn);
+  if (troubled) detect_troubled<true>(a, e, avg, sm, sn);
   double* rec = nullptr;
   if (a.tape) {
     const int64_t pos = *a.tape_head + slot;
@@ -571,8 +571,8 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_block(LimitArgs a) {
     for (int i = threadIdx.x; i < nrow * (SUMM_STRIDE / 2); i += LBLOCK) {
       const int p = i / nrow, r = i - p * nrow;
       const double2 v = src[(int64_t)p * a.K + r0 + r];
-      srow[r][2 * p] = v.x;
-      srow[r][2 * p + 1] = v.y;
+      srow[r][summ_lo(p)] = v.x;
+      srow[r][summ_hi(p)] = v.y;
     }
     __syncthreads();
   }
