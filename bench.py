@@ -10,7 +10,11 @@ boundaries, CFL 0.5. The state is resident in HBM before the timed region.
 value = DOF-stage updates per second = 3 stages x 60 K DOF x steps / time.
 roofline = the stage kernel (RHS fused with the stage combination), with
 algorithmic bytes (16 + 24 + 24)/3 = 21.33 B per DOF per launch, over its mean
-launch duration from HIP events on the solver's stream.
+launch duration from HIP events on the solver's stream; also as a fraction of
+the streaming ceiling measured in the same run (STREAM copy and add over
+2 GiB arrays, weighted by the stages' read/write mix). fp64 = the
+same launches' algorithmic flops (about 8.8k per element for HLLC) against
+the 78.6 TFLOP/s FP64 peak.
 cpu_baseline = the NumPy oracle (a restatement of the reference's JAX path;
 the JAX path itself is not available) on all cores of this process's CPU
 affinity (at most 16), its element range split over worker processes, on a
@@ -40,8 +44,16 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "p=4 DOF-updates/s (fp64) & % HBM roofline; batched speedup; adjoint/fwd cost"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64, vector and matrix alike
 STAGE_BYTES_PER_DOF = (16.0 + 24.0 + 24.0) / 3.0
 RHS_BYTES_PER_DOF = 16.0
+# Algorithmic flops per element of one HLLC right-hand side (divisions and
+# square roots one flop each): volume contractions 2 * (2 * 15 * 15 * 4) =
+# 3600, lift 2 * 15 * 15 * 4 = 1800, pointwise fluxes and contravariant
+# components 44 * 15 = 660, interface fluxes 15 * (160 + 20) = 2700. A stage
+# adds the SSP-RK3 combination (at most 3 flops per DOF).
+RHS_FLOPS_PER_ELEMENT_HLLC = 3600 + 1800 + 660 + 15 * (160 + 20)
+STAGE_FLOPS_PER_ELEMENT_HLLC = RHS_FLOPS_PER_ELEMENT_HLLC + 3 * 60
 
 
 def log(msg: str) -> None:
@@ -214,6 +226,44 @@ def batched_bench(n_cases: int, steps: int, seed: int = 0) -> dict:
     return {"cases": n_cases, "seed": seed, "dof": dof, "steps_per_case": steps, "stage_block": sb,
             "batched_dof_stage_per_s": work / t_batch, "loop_dof_stage_per_s": work_loop / t_loop,
             "speedup_vs_loop": (work / t_batch) / (work_loop / t_loop)}
+
+
+def stream_ceiling(gib: int = 2, reps: int = 10) -> dict:
+    """Measured streaming ceilings of this GPU, in the read/write mixes of the
+    kernels they bound: STREAM copy (c = a: one read, one write, the
+    right-hand side's mix) and STREAM add (c = a + b: two reads, one write,
+    the mix of SSP-RK3 stages 2 and 3), each over ``gib`` GiB per array of
+    fp64 with torch's elementwise kernels, median of ``reps`` launches timed
+    with events on torch's current stream, the one the kernels run on."""
+    import torch
+
+    n = gib * (1 << 27)
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.ones(n, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def med_ms(fn) -> float:
+        for _ in range(3):
+            fn()
+        ms = []
+        for _ in range(reps):
+            ev0.record()
+            fn()
+            ev1.record()
+            ev1.synchronize()
+            ms.append(ev0.elapsed_time(ev1))
+        return float(np.median(ms))
+
+    t_copy = med_ms(lambda: c.copy_(a))
+    t_add = med_ms(lambda: torch.add(a, b, out=c))
+    del a, b, c
+    torch.cuda.empty_cache()
+    out = {"copy_gbs": 2.0 * n * 8 / (t_copy * 1e-3) / 1e9, "add_gbs": 3.0 * n * 8 / (t_add * 1e-3) / 1e9,
+           "copy_ms": t_copy, "add_ms": t_add, "gib_per_array": gib,
+           "kernels": "torch Tensor.copy_ and torch.add(out=) on fp64, device resident"}
+    log(f"stream ceilings ({gib} GiB fp64 arrays): copy {out['copy_gbs']:.0f} GB/s, add {out['add_gbs']:.0f} GB/s")
+    return out
 
 
 def setup_bench() -> dict:
@@ -574,6 +624,15 @@ def main() -> int:
 
     traffic = traffic_from_profile("stage_kernel")
     rhs_traffic = traffic_from_profile("rhs_kernel")
+    stream = stream_ceiling()
+    # The stage launches' read/write mix: stage 1 reads one state and writes
+    # one, stages 2 and 3 read two and write one; their byte-weighted ceiling.
+    stage_ceiling = (16.0 + 24.0 + 24.0) / (16.0 / stream["copy_gbs"] + 2 * 24.0 / stream["add_gbs"])
+    stage_tflops = STAGE_FLOPS_PER_ELEMENT_HLLC * K / (stage_ms * 1e-3) / 1e12
+    rhs_tflops = RHS_FLOPS_PER_ELEMENT_HLLC * K / (rhs_ms.value * 1e-3) / 1e12
+    log(f"stage kernel {stage_gbs / stage_ceiling:.1%} of the measured streaming ceiling ({stage_ceiling:.0f} GB/s), "
+        f"FP64 {stage_tflops:.1f} TFLOP/s ({stage_tflops / FP64_PEAK_TFLOPS:.1%} of {FP64_PEAK_TFLOPS}); "
+        f"rhs {rhs_gbs / stream['copy_gbs']:.1%} of copy, {rhs_tflops:.1f} TFLOP/s")
     line = {
         "metric": METRIC,
         "value": value,
@@ -605,7 +664,17 @@ def main() -> int:
             "kernel": "k_stage4<HLLC> (RHS + SSP-RK3 combine + limiter summary, four lanes per element), mean of the 3 stage launches",
             "algorithmic_bytes_per_launch": STAGE_BYTES_PER_DOF * dof,
             "mean_launch_ms": stage_ms,
+            "measured_ceiling_gbs": stage_ceiling,
+            "frac_of_measured_ceiling": stage_gbs / stage_ceiling,
         },
+        "fp64": {
+            "achieved": stage_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": stage_tflops / FP64_PEAK_TFLOPS,
+            "flops_per_element_stage": STAGE_FLOPS_PER_ELEMENT_HLLC,
+            "rhs_achieved": rhs_tflops, "rhs_frac": rhs_tflops / FP64_PEAK_TFLOPS,
+            "flops_per_element_rhs": RHS_FLOPS_PER_ELEMENT_HLLC,
+        },
+        "stream_ceiling": stream,
         "rhs_roofline": {
             "bound": "hbm", "achieved": rhs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_gbs / HBM_PEAK_GBS,
             "kernel": "k_stage4<HLLC, RHS only>", "traffic": rhs_traffic, "traffic_source": TRAFFIC_SOURCE,
@@ -613,6 +682,7 @@ def main() -> int:
             "mean_launch_ms": rhs_ms.value,
             "median_launch_ms": rhs_med,
             "median_launch_ms_perturbed_input": rhs_med_pt,
+            "frac_of_measured_ceiling": rhs_gbs / stream["copy_gbs"],
         },
         "limiter_mean_launch_ms": limit_ms,
         "cpu_baseline": None,
