@@ -196,28 +196,6 @@ __device__ __forceinline__ void store4(double* p, const double v[4]) {
   reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
 }
 
-__device__ __forceinline__ void load_geo(const double* geo, int64_t e, ElemGeo& g) {
-  double t[16];
-  const double2* p = reinterpret_cast<const double2*>(geo + e * GEO_STRIDE);
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const double2 v = p[k];
-    t[2 * k] = v.x;
-    t[2 * k + 1] = v.y;
-  }
-  g.rx = t[0];
-  g.ry = t[1];
-  g.sx = t[2];
-  g.sy = t[3];
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    g.nx[f] = t[4 + f];
-    g.ny[f] = t[7 + f];
-    g.fs[f] = t[10 + f];
-    g.he[f] = t[13 + f];
-  }
-}
-
 // Face frame of an affine triangle from its inverse-Jacobian entries. The
 // reference faces s = -1, r + s = 0 and r = -1 map to the physical edge
 // vectors 2J (sy, -sx), 2J (-ry - sy, rx + sx) and 2J (ry, -rx), J the volume
@@ -225,10 +203,11 @@ __device__ __forceinline__ void load_geo(const double* geo, int64_t e, ElemGeo& 
 // -(rx, ry) and sigma the sign of J: outward unit normal sigma g_f / |g_f|,
 // face scale 0.5 |e_f| / J = sigma |g_f|, half edge length 0.5 |e_f| = |g_f| |J|.
 // The geometry row holds the same quantities computed from the vertices
-// (mesh.py:216-243, dg.py:99-101); the two agree to a few ulps. The
-// right-hand-side kernel computes them instead of loading them: nine of the
-// row's sixteen doubles, loaded by every lane of an element, are a third of
-// the data its loads return.
+// (mesh.py:216-243, dg.py:99-101); the two agree to a few ulps. The stage
+// and adjoint kernels compute them instead of loading them: nine of the
+// row's sixteen doubles, loaded by every lane of an element, were a third of
+// the data the four-lane kernel's loads return. The limiter keeps reading
+// the row's normals (slip-wall ghost averages).
 __device__ __forceinline__ void face_frame(double rx, double ry, double sx, double sy, int f, double& nx, double& ny,
                                            double& fs) {
   const double gx = f == 0 ? -sx : (f == 1 ? rx + sx : -rx);
@@ -239,6 +218,23 @@ __device__ __forceinline__ void face_frame(double rx, double ry, double sx, doub
   nx = gx * (ccw ? il : -il);
   ny = gy * (ccw ? il : -il);
   fs = ccw ? len : -len;
+}
+
+// The affine part of the geometry row (rx, ry, sx, sy) and the face frames
+// derived from it (face_frame; half edge length = face scale x J).
+__device__ __forceinline__ void load_geo_affine(const double* geo, int64_t e, ElemGeo& g) {
+  const double2* p = reinterpret_cast<const double2*>(geo + e * GEO_STRIDE);
+  const double2 a = p[0], b = p[1];
+  g.rx = a.x;
+  g.ry = a.y;
+  g.sx = b.x;
+  g.sy = b.y;
+  const double jac = frcp(g.rx * g.sy - g.ry * g.sx);
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    face_frame(g.rx, g.ry, g.sx, g.sy, f, g.nx[f], g.ny[f], g.fs[f]);
+    g.he[f] = g.fs[f] * jac;
+  }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD one
