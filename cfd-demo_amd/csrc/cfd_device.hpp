@@ -118,6 +118,10 @@ struct StageArgs {
   double dt;
   BatchView bv;  // batched solves only
   int blk0 = 0;  // batched one-flux launches: first element block of the launch
+  // Row stride of geo in doubles: the stage kernels read only rx, ry, sx, sy,
+  // from the context's compact K x 4 copy (stride 4) or from the full
+  // geometry rows (GEO_STRIDE) of caller-owned tables.
+  int geo_stride = GEO_STRIDE;
 };
 
 struct LimitArgs {
@@ -220,10 +224,11 @@ __device__ __forceinline__ void face_frame(double rx, double ry, double sx, doub
   fs = ccw ? len : -len;
 }
 
-// The affine part of the geometry row (rx, ry, sx, sy) and the face frames
-// derived from it (face_frame; half edge length = face scale x J).
-__device__ __forceinline__ void load_geo_affine(const double* geo, int64_t e, ElemGeo& g) {
-  const double2* p = reinterpret_cast<const double2*>(geo + e * GEO_STRIDE);
+// The affine part of a geometry row (rx, ry, sx, sy; rows of `stride`
+// doubles) and the face frames derived from it (face_frame; half edge length
+// = face scale x J).
+__device__ __forceinline__ void load_geo_affine(const double* geo, int stride, int64_t e, ElemGeo& g) {
+  const double2* p = reinterpret_cast<const double2*>(geo + e * stride);
   const double2 a = p[0], b = p[1];
   g.rx = a.x;
   g.ry = a.y;

@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
       for (int i = 0; i < NP; ++i) load4(a.q + ((int64_t)i * K + e) * 4, acc[i]);
     } else {
       ElemGeo g;
-      load_geo_affine(a.geo, e, g);
+      load_geo_affine(a.geo, a.geo_stride, e, g);
       const int4 cn = reinterpret_cast<const int4*>(a.conn)[e];
       const int nbr[3] = {cn.x, cn.y, cn.z};
       const uint32_t code = (uint32_t)cn.w;
