@@ -318,10 +318,15 @@ __device__ __forceinline__ void block_reduce_bflux(const double bf[4], double* o
   double v[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) v[c] = bf[c];
+  // Most waves hold no boundary face; their sums are +0 (every lane starts at
+  // +0 and only boundary slots add), so they skip the butterfly's 48 lane
+  // shuffles. A NaN counts as nonzero and still goes through.
+  if (__ballot(v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0) != 0) {
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
+    for (int off = 32; off >= 1; off >>= 1)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) v[c] = v[c] + __shfl_xor(v[c], off, 64);
+      for (int c = 0; c < 4; ++c) v[c] = v[c] + __shfl_xor(v[c], off, 64);
+  }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
