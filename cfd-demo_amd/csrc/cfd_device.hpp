@@ -279,8 +279,43 @@ __device__ __forceinline__ double part_combine(int c, double a, double b) {
   return a + b;
 }
 
+// A double moved across lanes by a DPP control: quad permutations (0x00-0xFF)
+// and rotations within rows of 16 lanes (0x121-0x12F) are VALU moves, with no
+// trip through the LDS crossbar that a lane shuffle takes.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Fixed-order reduction of one column over a wave; lane 0 ends with
+// ((q0 q1)(q2 q3))((q4 q5)(q6 q7)) ... in a balanced tree over the sixteen
+// quads. Within a row of 16 lanes: partner lane ^1 and ^2 (quad
+// permutations), then the lane 4 and 8 places on (row rotations, which meet
+// the butterfly's partners on the lanes lane 0 depends on); across rows:
+// lane shuffles ^16 and ^32.
+template <int C>
+__device__ __forceinline__ double wave_reduce_col(double v) {
+  v = part_combine(C, v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = part_combine(C, v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = part_combine(C, v, dpp_f64<0x124>(v));  // row_ror 4
+  v = part_combine(C, v, dpp_f64<0x128>(v));  // row_ror 8
+  v = part_combine(C, v, __shfl_xor(v, 16, 64));
+  return part_combine(C, v, __shfl_xor(v, 32, 64));
+}
+
+template <int C = 0>
+__device__ __forceinline__ void wave_reduce_row(double v[PART_COLS]) {
+  if constexpr (C < PART_COLS) {
+    v[C] = wave_reduce_col<C>(v[C]);
+    wave_reduce_row<C + 1>(v);
+  }
+}
+
 // Block-wide deterministic reduction of one partial row (NT threads): a fixed
-// butterfly inside each wave, then wave results in order through LDS.
+// tree inside each wave (wave_reduce_col), then wave results in order through
+// LDS.
 template <int NT>
 __device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], double* out) {
   constexpr int NW = NT / 64;
@@ -288,10 +323,7 @@ __device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], do
   double v[PART_COLS];
 #pragma unroll
   for (int c = 0; c < PART_COLS; ++c) v[c] = row[c];
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < PART_COLS; ++c) v[c] = part_combine(c, v[c], __shfl_xor(v[c], off, 64));
+  wave_reduce_row(v);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
@@ -308,9 +340,8 @@ __device__ __forceinline__ void block_reduce_row(const double row[PART_COLS], do
 }
 
 // The stage kernels' partial row: only the four boundary-flux sums vary, the
-// other columns are neutral. Same butterfly and wave order as
-// block_reduce_row on those columns (identical bits), without the work on the
-// constant ones.
+// other columns are neutral. Same tree and wave order as block_reduce_row on
+// those columns (identical bits), without the work on the constant ones.
 template <int NT>
 __device__ __forceinline__ void block_reduce_bflux(const double bf[4], double* out) {
   constexpr int NW = NT / 64;
@@ -322,10 +353,10 @@ __device__ __forceinline__ void block_reduce_bflux(const double bf[4], double* o
   // +0 and only boundary slots add), so they skip the butterfly's 48 lane
   // shuffles. A NaN counts as nonzero and still goes through.
   if (__ballot(v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0 || v[3] != 0.0) != 0) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[c] = v[c] + __shfl_xor(v[c], off, 64);
+    v[0] = wave_reduce_col<0>(v[0]);
+    v[1] = wave_reduce_col<1>(v[1]);
+    v[2] = wave_reduce_col<2>(v[2]);
+    v[3] = wave_reduce_col<3>(v[3]);
   }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
