@@ -122,6 +122,10 @@ struct StageArgs {
   // from the context's compact K x 4 copy (stride 4) or from the full
   // geometry rows (GEO_STRIDE) of caller-owned tables.
   int geo_stride = GEO_STRIDE;
+  // Device-controlled runs (nullptr: off): the step's dt, read from device
+  // memory; a value that is not > 0 marks a step enqueued after the run
+  // stopped, and the kernel returns at once.
+  const double* dtp = nullptr;
 };
 
 struct LimitArgs {
@@ -159,6 +163,28 @@ struct LimitArgs {
   // Repair pass (nullptr: off): the worklist length of this pass is stored
   // here (the adjoint's recomputation skips passes that repaired nothing).
   int* count_out = nullptr;
+  const double* dtp = nullptr;  // as StageArgs::dtp
+};
+
+// A step of a device-controlled run that the control kernel has stopped.
+__device__ __forceinline__ bool step_stopped(const double* dtp) { return dtp && !(*dtp > 0.0); }
+
+// EulerSolver.run's loop state (solver.py:167-216) for runs whose steps are
+// enqueued ahead of the host: k_run_control applies the loop body after each
+// step, so the host reads this block once per batch of steps instead of a
+// report per step.
+enum RunStatus : int { RUN_GO = 0, RUN_DONE = 1, RUN_RETRY = 2, RUN_INVALID_DT = 3, RUN_REJECTED = 4 };
+struct RunCtl {
+  double dt;          // dt of the next step (the step kernels read it); 0 while stopped
+  double retry_dt;    // RUN_RETRY: the halved dt of the next attempt
+  double next_dt;     // RUN_DONE: cfl x the last step's CFL candidate
+  double t, final_time, cfl;
+  double fint[4];     // boundary-flux integral
+  double mind, minp;  // running minima (Python min semantics)
+  double fail_time, fail_dt;
+  double last_dt, last_bflux[4], last_rmin, last_pmin;  // the last accepted step and its report
+  int64_t steps, rejected, max_troubled, max_steps, last_troubled;
+  int status, attempt, max_rejections, reaches_end;
 };
 
 // Tape record of one limiter worklist element (TAPE_STRIDE doubles):

@@ -56,6 +56,12 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
   const double gm1 = gamma - 1.0;
   double dt = a.dt;
   int flux_rt = FLUX;
+  if constexpr (FLUX != FLUX_DYNAMIC) {
+    if (a.dtp) {
+      dt = *a.dtp;
+      if (!(dt > 0.0)) return;
+    }
+  }
   if constexpr (FLUX == FLUX_DYNAMIC) {
     const int cs = a.bv.lcase[e0 >> 8];
     dt = a.bv.dt[cs];
@@ -286,6 +292,7 @@ __device__ __forceinline__ void append_work(int* count, int* list, bool want, in
 }
 
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
+  if (step_stopped(a.dtp)) return;
   const int blk = listed_block(a.bv, xcd_block(blockIdx.x, gridDim.x), 1);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
@@ -318,6 +325,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
 
 // Unlimited runs (no slopes, so no neighbour rows and no batched cases).
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect_direct(LimitArgs a) {
+  if (step_stopped(a.dtp)) return;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
@@ -509,6 +517,7 @@ __device__ __forceinline__ void repair_group(const LimitArgs& a, int entry, int6
 // four-quadrant run (a quarter of the elements repaired per pass) the limiter
 // went from 0.495 to 0.415 ms per stage. Five waves spilled and were slower.
 __global__ void __launch_bounds__(FBLOCK) __attribute__((amdgpu_waves_per_eu(4))) k_limit_fix(LimitArgs a) {
+  if (step_stopped(a.dtp)) return;
   __shared__ double qs[FBLOCK / RG][NP][4];
   __shared__ double ds[FBLOCK / RG][12];
   double row[PART_COLS];
@@ -554,6 +563,7 @@ __global__ void __launch_bounds__(FBLOCK) __attribute__((amdgpu_waves_per_eu(4))
 // element's nodes and detection reads only summaries, so no grid-wide barrier
 // is needed between the two; one partial row per block.
 __global__ void __launch_bounds__(LBLOCK) k_limit_block(LimitArgs a) {
+  if (step_stopped(a.dtp)) return;
   const int blk = xcd_block(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
@@ -661,7 +671,8 @@ __global__ void __launch_bounds__(LBLOCK) k_integrate(DtArgs a) {
 // block-completion counter and, if given, the limiter worklist counter).
 __global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ part, int nrows,
                                                     double* __restrict__ chunk, int* done, int* work_count,
-                                                    double* __restrict__ out) {
+                                                    double* __restrict__ out, const double* dtp) {
+  if (step_stopped(dtp)) return;
   __shared__ int last;
   const int per = (nrows + gridDim.x - 1) / gridDim.x;
   const int r0 = blockIdx.x * per;
@@ -697,6 +708,88 @@ __global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ 
     *done = 0;
     if (work_count) *work_count = 0;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Device-controlled runs: EulerSolver.run's loop body (solver.py:167-216)
+// after each step, from the step's report rows (the host loop's fill_report
+// and acceptance logic, operation for operation).
+// ---------------------------------------------------------------------------
+__global__ void k_run_control(RunCtl* __restrict__ r, const double* __restrict__ rep) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || r->status != RUN_GO) return;
+  const double dt = r->dt;
+  const double* b0 = rep;
+  const double* b1 = rep + PART_STRIDE;
+  const double* b2 = rep + 2 * PART_STRIDE;
+  double bf[4];
+  for (int k = 0; k < 4; ++k) bf[k] = dt * (b0[k] / 6.0 + b1[k] / 6.0 + 2.0 * b2[k] / 3.0);
+  bool adm = true;
+  double rmin = b0[COL_RMIN], pmin = b0[COL_PMIN], tmax = b0[COL_TROUBLED];
+  for (int s = 0; s < 3; ++s) {
+    const double* row = rep + s * PART_STRIDE;
+    adm = adm && (row[COL_ADM] >= 1.0);
+    if (row[COL_RMIN] != row[COL_RMIN] || row[COL_RMIN] < rmin) rmin = row[COL_RMIN];
+    if (row[COL_PMIN] != row[COL_PMIN] || row[COL_PMIN] < pmin) pmin = row[COL_PMIN];
+    if (row[COL_TROUBLED] > tmax) tmax = row[COL_TROUBLED];
+  }
+  if (!adm) {
+    // Rejected: retry from the same input with half the step, up to
+    // max_rejections times. The host re-enqueues from that input.
+    ++r->rejected;
+    if (r->attempt >= r->max_rejections) {
+      r->status = RUN_REJECTED;
+      r->fail_time = r->t;
+      r->dt = 0.0;
+      return;
+    }
+    ++r->attempt;
+    r->reaches_end = 0;
+    r->retry_dt = dt * 0.5;
+    r->status = RUN_RETRY;
+    r->dt = 0.0;
+    return;
+  }
+  r->t = r->reaches_end ? r->final_time : r->t + dt;
+  ++r->steps;
+  for (int k = 0; k < 4; ++k) r->fint[k] += bf[k];
+  r->mind = rmin < r->mind ? rmin : r->mind;
+  r->minp = pmin < r->minp ? pmin : r->minp;
+  const int64_t tr = (int64_t)tmax;
+  if (tr > r->max_troubled) r->max_troubled = tr;
+  r->attempt = 0;
+  r->last_dt = dt;
+  for (int k = 0; k < 4; ++k) r->last_bflux[k] = bf[k];
+  r->last_rmin = rmin;
+  r->last_pmin = pmin;
+  r->last_troubled = tr;
+  // Head of the next iteration.
+  const double dt_next = r->cfl * b2[COL_DT];
+  if (!(r->t < r->final_time) || (r->max_steps >= 0 && r->steps >= r->max_steps)) {
+    r->status = RUN_DONE;
+    r->next_dt = dt_next;
+    r->dt = 0.0;
+    return;
+  }
+  double d = dt_next;
+  if (!isfinite(d) || d <= 0.0) {
+    r->status = RUN_INVALID_DT;
+    r->fail_time = r->t;
+    r->fail_dt = d;
+    r->dt = 0.0;
+    return;
+  }
+  const double remaining = r->final_time - r->t;
+  r->reaches_end = d >= remaining;
+  if (r->reaches_end) d = remaining;
+  r->dt = d;
+}
+
+// Resume after a rejected attempt: the next attempt runs with the halved dt.
+__global__ void k_run_resume(RunCtl* __restrict__ r) {
+  if (threadIdx.x != 0 || r->status != RUN_RETRY) return;
+  r->status = RUN_GO;
+  r->dt = r->retry_dt;
 }
 
 // Pointwise evaluation of an interface flux on n face states (API tests).

@@ -1,0 +1,134 @@
+"""The device-controlled run loop against the per-step host loop and the oracle.
+
+``EulerSolver.run`` (``solver.py:167-216``) and ``cfd_state_advance`` enqueue
+steps ahead of the host: after every step ``k_run_control`` applies the loop
+body on the GPU (acceptance, the halved retry of a rejected attempt, the
+clamp to the final time, the boundary-flux integral and the minima), and the
+host reads the loop state once per batch of up to 64 steps. The per-step host
+loop (``CFDHIP_HOST_LOOP=1``) runs the same kernels with a report read after
+every step, so the two must agree bit for bit; both are also compared with the
+oracle's ``run``, including runs with many rejected attempts.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from pairs import make_pair, perturbed_vertices, sod_bcs
+
+pytestmark = pytest.mark.gpu
+
+
+def _sod_state(g):
+    left = g.geometry.centroid[:, 0] < 0.5
+    q = np.empty(g.state_shape)
+    q[:, left] = [1.0, 0.0, 0.0, 2.5]
+    q[:, ~left] = [0.125, 0.0, 0.0, 0.25]
+    return q
+
+
+def _both_loops(monkeypatch, make, run):
+    """run(solver) with the per-step host loop, then with the device loop
+    (the loop kind is fixed when a solver's context is created)."""
+    monkeypatch.setenv("CFDHIP_HOST_LOOP", "1")
+    host = run(make())
+    monkeypatch.delenv("CFDHIP_HOST_LOOP")
+    dev = run(make())
+    return dev, host
+
+
+def _assert_same_run(a, b):
+    assert a.steps == b.steps and a.rejected_steps == b.rejected_steps
+    assert a.time == b.time
+    assert a.max_troubled_cells == b.max_troubled_cells
+    assert np.array_equal(np.asarray(a.boundary_flux_integral), np.asarray(b.boundary_flux_integral))
+    assert a.min_density == b.min_density and a.min_pressure == b.min_pressure
+    assert np.array_equal(a.state, b.state)
+
+
+@pytest.mark.parametrize("cfl, final_time", [(0.5, 0.05), (32.0, 0.2), (100.0, 0.2)])
+def test_device_loop_equals_host_loop_and_oracle(monkeypatch, cfl, final_time):
+    """Sod tube 20x2, Rusanov + minmod. At CFL 32 the oracle takes 17 steps
+    and rejects 14 attempts, at CFL 100 16 steps and 33 rejected attempts
+    (several retries of one step in a row): every retry and the exact end
+    time go through the device control."""
+    v, e, b = perturbed_vertices(20, 2, amplitude=0.0, seed=1, y_range=(0.0, 0.1))
+    make = lambda: make_pair(v, e, b, sod_bcs(), cfl=cfl, max_rejections=10)[0]
+    g0, o = make_pair(v, e, b, sod_bcs(), cfl=cfl, max_rejections=10)
+    q = _sod_state(g0)
+    print(f"case sod 20x2 cfl {cfl} t {final_time}", flush=True)
+    dev, host = _both_loops(monkeypatch, make, lambda s: s.run(q, final_time))
+    _assert_same_run(dev, host)
+    ref = o.run(q, final_time)
+    assert dev.steps == ref.steps and dev.rejected_steps == ref.rejected_steps
+    assert dev.time == ref.time == final_time
+    assert dev.max_troubled_cells == ref.max_troubled_cells
+    # At CFL 0.5 the states agree to 1e-10. Steps of 32-100 times the stable
+    # CFL step amplify the last-bit differences between the GPU's and the
+    # oracle's arithmetic (the host loop lands on the device loop's bits, so
+    # this is not the control path): there the bound is 1e-6.
+    tol = 1e-10 if cfl <= 1.0 else 1e-6
+    assert np.abs(dev.state - ref.state).max() <= tol * np.abs(ref.state).max()
+    if cfl > 1.0:
+        assert dev.rejected_steps > 0
+
+
+def test_device_loop_max_steps(monkeypatch):
+    """max_steps stops the loop before the final time, as the host loop does
+    (the same steps, time and state)."""
+    v, e, b = perturbed_vertices(20, 2, amplitude=0.0, seed=1, y_range=(0.0, 0.1))
+    make = lambda: make_pair(v, e, b, sod_bcs(), cfl=0.5)[0]
+    q = _sod_state(make())
+    print("case sod 20x2 max_steps 7", flush=True)
+    dev, host = _both_loops(monkeypatch, make, lambda s: s.run(q, 0.2, max_steps=7))
+    _assert_same_run(dev, host)
+    assert dev.steps == 7 and dev.time < 0.2
+
+
+def test_device_loop_long_run_across_batches(monkeypatch):
+    """The four-quadrant problem (24x24, HLLC + MC) for several hundred steps:
+    many batches of enqueued steps, the last one cut short at the final time."""
+    from cfd_demo_amd.cases import four_quadrant
+
+    make = lambda: four_quadrant(24, flux="hllc", limiter="mc").solver
+    q = np.ascontiguousarray(four_quadrant(24, flux="hllc", limiter="mc").initial_state)
+    print("case four-quadrant 24x24 to t 0.1", flush=True)
+    dev, host = _both_loops(monkeypatch, make, lambda s: s.run(q, 0.1))
+    _assert_same_run(dev, host)
+    assert dev.steps > 100 and dev.time == 0.1
+
+
+def test_state_advance_device_equals_host(monkeypatch):
+    """cfd_state_advance (the resident-state stepping the benchmark times):
+    37 steps in three calls, device loop against host loop, bit for bit in
+    state, last dt and last step report."""
+    from cfd_demo_amd import _lib
+    from cfd_demo_amd.cases import four_quadrant
+
+    case = four_quadrant(24, flux="hllc", limiter="mc")
+    q0 = np.ascontiguousarray(case.initial_state)
+    lib = _lib.load()
+
+    def advance(solver):
+        ctx = solver._context()
+        _lib.check(lib.cfd_state_upload(ctx, _lib.dptr(q0)))
+        dts, reps = [], []
+        for n in (1, 20, 16):
+            dt = C.c_double(0.0)
+            rep = _lib.CfdStepReport()
+            _lib.check(lib.cfd_state_advance(ctx, n, C.byref(dt), C.byref(rep)))
+            dts.append(dt.value)
+            reps.append((tuple(rep.boundary_flux), rep.admissible, rep.min_density, rep.min_pressure,
+                         rep.troubled_cells))
+        out = np.empty_like(q0)
+        _lib.check(lib.cfd_state_download(ctx, _lib.dptr(out)))
+        return out, dts, reps
+
+    print("case state_advance four-quadrant 24x24, 37 steps", flush=True)
+    make = lambda: four_quadrant(24, flux="hllc", limiter="mc").solver
+    (qd, dtd, repd), (qh, dth, reph) = _both_loops(monkeypatch, make, advance)
+    assert np.array_equal(qd, qh)
+    assert dtd == dth and repd == reph
