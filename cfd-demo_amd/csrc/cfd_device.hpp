@@ -276,6 +276,23 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Correctly rounded x / 3 without a division (a division is eleven
+// dependent instructions, and the last stage takes fifteen per lane; this is
+// seven). q = RN(x RN(1/3)) is within one ulp of x / 3, so the remainder
+// x - 3q is exact as an FMA (also for subnormal x, where x and 3q are
+// multiples of the same quantum, and near overflow, since the FMA does not
+// round 3q), and Markstein's correction RN(q + r RN(1/3)) is RN(x / 3): x / 3
+// is never a rounding midpoint, which would take a 55-bit x. copysign keeps
+// the sign of a zero x; a NaN remainder (x infinite or NaN) returns q.
+__device__ __forceinline__ double div3(double x) {
+#pragma clang fp contract(off)
+  constexpr double third = 0x1.5555555555555p-2;  // RN(1/3)
+  const double q = x * third;
+  const double r = fma(-q, 3.0, x);
+  const double res = copysign(fma(r, third, q), x);
+  return r == r ? res : q;
+}
+
 // SSP-RK3 stage combinations with the reference's association
 // (time_integration.py:70,72,74); contraction is off so the bits match.
 template <int MODE>
@@ -286,7 +303,7 @@ __device__ __forceinline__ double rk3_combine(double q0, double qi, double dt, d
   } else if constexpr (MODE == MODE_STAGE2) {
     return 0.75 * q0 + 0.25 * (qi + dt * rhs);
   } else {
-    return q0 / 3.0 + 2.0 / 3.0 * (qi + dt * rhs);
+    return div3(q0) + 2.0 / 3.0 * (qi + dt * rhs);
   }
 }
 
