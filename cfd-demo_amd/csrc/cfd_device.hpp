@@ -170,7 +170,7 @@ struct LimitArgs {
 __device__ __forceinline__ bool step_stopped(const double* dtp) { return dtp && !(*dtp > 0.0); }
 
 // EulerSolver.run's loop state (solver.py:167-216) for runs whose steps are
-// enqueued ahead of the host: k_run_control applies the loop body after each
+// enqueued ahead of the host: run_control applies the loop body after each
 // step, so the host reads this block once per batch of steps instead of a
 // report per step.
 enum RunStatus : int { RUN_GO = 0, RUN_DONE = 1, RUN_RETRY = 2, RUN_INVALID_DT = 3, RUN_REJECTED = 4 };

@@ -669,55 +669,15 @@ __global__ void __launch_bounds__(LBLOCK) k_integrate(DtArgs a) {
 // block reduces a fixed contiguous chunk; the last block to finish reduces
 // the chunk results in order, writes out, and re-arms the counters (the
 // block-completion counter and, if given, the limiter worklist counter).
-__global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ part, int nrows,
-                                                    double* __restrict__ chunk, int* done, int* work_count,
-                                                    double* __restrict__ out, const double* dtp) {
-  if (step_stopped(dtp)) return;
-  __shared__ int last;
-  const int per = (nrows + gridDim.x - 1) / gridDim.x;
-  const int r0 = blockIdx.x * per;
-  const int r1 = min(nrows, r0 + per);
-  double row[PART_COLS];
-  part_neutral(row);
-  for (int r = r0 + threadIdx.x; r < r1; r += LBLOCK) {
-    const double* p = part + (int64_t)r * PART_STRIDE;
-#pragma unroll
-    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
-  }
-  if (gridDim.x == 1) {
-    // Small meshes reduce in one block: its result is the report row.
-    block_reduce_row<LBLOCK>(row, out);
-    if (threadIdx.x == 0 && work_count) *work_count = 0;
-    return;
-  }
-  block_reduce_row<LBLOCK>(row, chunk + (int64_t)blockIdx.x * PART_STRIDE);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = (atomicAdd(done, 1) == (int)gridDim.x - 1);
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  part_neutral(row);
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += LBLOCK) {
-    const volatile double* p = chunk + (int64_t)b * PART_STRIDE;
-#pragma unroll
-    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
-  }
-  block_reduce_row<LBLOCK>(row, out);
-  if (threadIdx.x == 0) {
-    *done = 0;
-    if (work_count) *work_count = 0;
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Device-controlled runs: EulerSolver.run's loop body (solver.py:167-216)
 // after each step, from the step's report rows (the host loop's fill_report
-// and acceptance logic, operation for operation).
+// and acceptance logic, operation for operation). Run by the thread of the
+// last stage's report reduction that writes the report row.
 // ---------------------------------------------------------------------------
-__global__ void k_run_control(RunCtl* __restrict__ r, const double* __restrict__ rep) {
+__device__ __forceinline__ void run_control(RunCtl* __restrict__ r, const double* __restrict__ rep) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0 || r->status != RUN_GO) return;
+  if (r->status != RUN_GO) return;
   const double dt = r->dt;
   const double* b0 = rep;
   const double* b1 = rep + PART_STRIDE;
@@ -790,6 +750,50 @@ __global__ void k_run_resume(RunCtl* __restrict__ r) {
   if (threadIdx.x != 0 || r->status != RUN_RETRY) return;
   r->status = RUN_GO;
   r->dt = r->retry_dt;
+}
+
+__global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ part, int nrows,
+                                                    double* __restrict__ chunk, int* done, int* work_count,
+                                                    double* __restrict__ out, const double* dtp, RunCtl* ctl,
+                                                    const double* rep) {
+  if (step_stopped(dtp)) return;
+  __shared__ int last;
+  const int per = (nrows + gridDim.x - 1) / gridDim.x;
+  const int r0 = blockIdx.x * per;
+  const int r1 = min(nrows, r0 + per);
+  double row[PART_COLS];
+  part_neutral(row);
+  for (int r = r0 + threadIdx.x; r < r1; r += LBLOCK) {
+    const double* p = part + (int64_t)r * PART_STRIDE;
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+  }
+  if (gridDim.x == 1) {
+    // Small meshes reduce in one block: its result is the report row.
+    block_reduce_row<LBLOCK>(row, out);
+    if (threadIdx.x == 0 && work_count) *work_count = 0;
+    if (threadIdx.x == 0 && ctl) run_control(ctl, rep);
+    return;
+  }
+  block_reduce_row<LBLOCK>(row, chunk + (int64_t)blockIdx.x * PART_STRIDE);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = (atomicAdd(done, 1) == (int)gridDim.x - 1);
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  part_neutral(row);
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += LBLOCK) {
+    const volatile double* p = chunk + (int64_t)b * PART_STRIDE;
+#pragma unroll
+    for (int c = 0; c < PART_COLS; ++c) row[c] = part_combine(c, row[c], p[c]);
+  }
+  block_reduce_row<LBLOCK>(row, out);
+  if (threadIdx.x == 0) {
+    *done = 0;
+    if (work_count) *work_count = 0;
+    if (ctl) run_control(ctl, rep);
+  }
 }
 
 // Pointwise evaluation of an interface flux on n face states (API tests).

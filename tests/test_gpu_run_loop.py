@@ -1,10 +1,11 @@
 """The device-controlled run loop against the per-step host loop and the oracle.
 
 ``EulerSolver.run`` (``solver.py:167-216``) and ``cfd_state_advance`` enqueue
-steps ahead of the host: after every step ``k_run_control`` applies the loop
-body on the GPU (acceptance, the halved retry of a rejected attempt, the
-clamp to the final time, the boundary-flux integral and the minima), and the
-host reads the loop state once per batch of up to 64 steps. The per-step host
+steps ahead of the host: after every step ``run_control`` (in the step's last
+report reduction) applies the loop body on the GPU (acceptance, the halved
+retry of a rejected attempt, the clamp to the final time, the boundary-flux
+integral and the minima), and the host reads the loop state once per batch of
+up to 64 steps. The per-step host
 loop (``CFDHIP_HOST_LOOP=1``) runs the same kernels with a report read after
 every step, so the two must agree bit for bit; both are also compared with the
 oracle's ``run``, including runs with many rejected attempts.
