@@ -665,10 +665,6 @@ __global__ void __launch_bounds__(LBLOCK) k_integrate(DtArgs a) {
   block_reduce_row<LBLOCK>(row, a.part + (int64_t)blk * PART_STRIDE);
 }
 
-// Deterministic reduction of nrows partial rows into out[PART_COLS]. Each
-// block reduces a fixed contiguous chunk; the last block to finish reduces
-// the chunk results in order, writes out, and re-arms the counters (the
-// block-completion counter and, if given, the limiter worklist counter).
 // ---------------------------------------------------------------------------
 // Device-controlled runs: EulerSolver.run's loop body (solver.py:167-216)
 // after each step, from the step's report rows (the host loop's fill_report
@@ -752,9 +748,16 @@ __global__ void k_run_resume(RunCtl* __restrict__ r) {
   r->dt = r->retry_dt;
 }
 
+// Deterministic reduction of nrows partial rows into out[PART_COLS]. Each
+// block reduces a fixed contiguous chunk; the last block to finish reduces
+// the chunk results in order, writes out, and re-arms the counters (the
+// block-completion counter and, if given, the limiter worklist counter).
+// Device-controlled runs: a stopped step's reduction returns at once (dtp),
+// and for the last stage (ctl) the thread that wrote out applies the loop
+// body to the report rows at rep (out is one of them, hence not restrict).
 __global__ void __launch_bounds__(LBLOCK) k_finalize(const double* __restrict__ part, int nrows,
                                                     double* __restrict__ chunk, int* done, int* work_count,
-                                                    double* __restrict__ out, const double* dtp, RunCtl* ctl,
+                                                    double* out, const double* dtp, RunCtl* ctl,
                                                     const double* rep) {
   if (step_stopped(dtp)) return;
   __shared__ int last;
