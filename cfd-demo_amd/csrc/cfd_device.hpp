@@ -286,6 +286,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// The same ranges, each walked from its end: the limiter's detection pass
+// reads the summaries the stage kernel wrote just before in the same per-XCD
+// order, so starting from the most recent ones finds more of them still in
+// the Infinity Cache (detection + repair 0.107 -> 0.100 ms per stage at 2M
+// elements). Partial rows stay indexed by block, so results are unchanged.
+__device__ __forceinline__ int xcd_block_rev(int b, int nb) {
+  const int x = b & 7;
+  const int q = nb >> 3, r = nb & 7;
+  const int len = q + (x < r ? 1 : 0);
+  return x * q + (x < r ? x : r) + (len - 1 - (b >> 3));
+}
+
 // Correctly rounded x / 3 without a division (a division is eleven
 // dependent instructions, and the last stage takes fifteen per lane; this is
 // seven). q = RN(x RN(1/3)) is within one ulp of x / 3, so the remainder

@@ -293,7 +293,8 @@ __device__ __forceinline__ void append_work(int* count, int* list, bool want, in
 
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
   if (step_stopped(a.dtp)) return;
-  const int blk = listed_block(a.bv, xcd_block(blockIdx.x, gridDim.x), 1);
+  const int blk = a.bv.lcase ? listed_block(a.bv, xcd_block(blockIdx.x, gridDim.x), 1)
+                             : xcd_block_rev(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
   part_neutral(row);
@@ -326,7 +327,7 @@ __global__ void __launch_bounds__(LBLOCK) k_limit_detect(LimitArgs a) {
 // Unlimited runs (no slopes, so no neighbour rows and no batched cases).
 __global__ void __launch_bounds__(LBLOCK) k_limit_detect_direct(LimitArgs a) {
   if (step_stopped(a.dtp)) return;
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int blk = xcd_block_rev(blockIdx.x, gridDim.x);
   const int64_t e = (int64_t)blk * LBLOCK + threadIdx.x;
   double row[PART_COLS];
   part_neutral(row);
