@@ -249,7 +249,10 @@ def split_quadrature(corners, breakpoints=(), n: int = QUAD_POINTS, lines=()) ->
     # Reference coordinates in the parent: invert its affine map.
     pc = corners[parent]
     j = np.stack([pc[:, 1] - pc[:, 0], pc[:, 2] - pc[:, 0]], -1)  # (M, 2, 2) columns
-    lam = np.linalg.solve(j[:, None], (pts - pc[:, None, 0])[..., None])[..., 0]  # barycentric 1, 2
+    d = pts - pc[:, None, 0]  # (M, n, 2)
+    det = (j[:, 0, 0] * j[:, 1, 1] - j[:, 0, 1] * j[:, 1, 0])[:, None]
+    lam = np.stack([(j[:, None, 1, 1] * d[..., 0] - j[:, None, 0, 1] * d[..., 1]) / det,
+                    (j[:, None, 0, 0] * d[..., 1] - j[:, None, 1, 0] * d[..., 0]) / det], -1)  # barycentric 1, 2
     return SplitQuadrature(pts[..., 0], pts[..., 1], weight, parent, 2.0 * lam - 1.0)
 
 
@@ -257,14 +260,24 @@ def monomial_powers(degree: int = 4):
     return np.array([(i, j) for i in range(degree + 1) for j in range(degree + 1 - i)])
 
 
+def _monomials(r, s, pw):
+    """r^i s^j for every (i, j) row of pw, by repeated products (shape r.shape + (len(pw),))."""
+    deg = int(pw.max())
+    rp, sp = [np.ones_like(r)], [np.ones_like(s)]
+    for _ in range(deg):
+        rp.append(rp[-1] * r)
+        sp.append(sp[-1] * s)
+    rp, sp = np.stack(rp, -1), np.stack(sp, -1)
+    return rp[..., pw[:, 0]] * sp[..., pw[:, 1]]
+
+
 def evaluate_nodal(state, r_nodes, s_nodes, quad: SplitQuadrature, degree: int = 4):
     """The nodal solution (Np, K, nv) at the quadrature points (M, n, nv)."""
     pw = monomial_powers(degree)
-    vand = r_nodes[:, None] ** pw[None, :, 0] * s_nodes[:, None] ** pw[None, :, 1]
+    vand = _monomials(r_nodes, s_nodes, pw)
     coef = np.linalg.solve(vand, state.reshape(state.shape[0], -1)).reshape(state.shape)  # (Np, K, nv)
-    r, s = quad.ref[..., 0], quad.ref[..., 1]
-    basis = r[..., None] ** pw[:, 0] * s[..., None] ** pw[:, 1]  # (M, n, Np)
-    return np.einsum("mnb,bmv->mnv", basis, coef[:, quad.parent, :])
+    basis = _monomials(quad.ref[..., 0], quad.ref[..., 1], pw)  # (M, n, Np)
+    return np.matmul(basis, coef.transpose(1, 0, 2)[quad.parent])  # (M, n, nv)
 
 
 def primitive(q, gamma):

@@ -35,6 +35,7 @@ def _integral(s, q):
     return np.einsum("ik,ikc->c", w, q)
 
 
+@pytest.mark.timeout(400)  # a whole batched run, then 256 exact-solution comparisons on the host
 def test_all_seeded_tubes_to_their_final_times():
     specs = random_shock_tube_specs(256, seed=0)
     cases = [shock_tube_case(s) for s in specs]
