@@ -133,3 +133,44 @@ def test_state_advance_device_equals_host(monkeypatch):
     (qd, dtd, repd), (qh, dth, reph) = _both_loops(monkeypatch, make, advance)
     assert np.array_equal(qd, qh)
     assert dtd == dth and repd == reph
+
+
+@pytest.mark.parametrize("final_time, max_steps", [(0.0, None), (-1.0, None), (0.05, 0)])
+def test_runs_that_take_no_step(monkeypatch, final_time, max_steps):
+    """solver.py:180: a final time at or before t = 0 (or max_steps = 0) takes
+    no step: the state comes back unchanged at t = 0 with the initial state's
+    minima and a zero flux integral, in both loops and in the oracle."""
+    v, e, b = perturbed_vertices(20, 2, amplitude=0.0, seed=1, y_range=(0.0, 0.1))
+    make = lambda: make_pair(v, e, b, sod_bcs(), cfl=0.5)[0]
+    g0, o = make_pair(v, e, b, sod_bcs(), cfl=0.5)
+    q = _sod_state(g0)
+    print(f"case no-step run t {final_time} max_steps {max_steps}", flush=True)
+    dev, host = _both_loops(monkeypatch, make, lambda s: s.run(q, final_time, max_steps=max_steps))
+    _assert_same_run(dev, host)
+    ref = o.run(q, final_time, max_steps=max_steps)
+    assert dev.steps == ref.steps == 0 and dev.rejected_steps == 0
+    assert dev.time == ref.time == 0.0
+    assert np.array_equal(dev.state, q)
+    assert np.array_equal(np.asarray(dev.boundary_flux_integral), np.zeros(4))
+    assert dev.min_density == pytest.approx(ref.min_density, rel=1e-14)
+    assert dev.min_pressure == pytest.approx(ref.min_pressure, rel=1e-14)
+
+
+def test_rejection_limit_in_both_loops(monkeypatch):
+    """A step rejected max_rejections + 1 times (CFL 1e6) raises the same
+    SolverError, with the same failure time, from the device loop and the
+    per-step host loop."""
+    import cfd_demo_amd as gpu
+
+    v, e, b = perturbed_vertices(20, 2, amplitude=0.0, seed=1, y_range=(0.0, 0.1))
+    make = lambda: make_pair(v, e, b, sod_bcs(), cfl=1e6, max_rejections=2)[0]
+    q = _sod_state(make())
+    print("case rejection limit, both loops", flush=True)
+
+    def run(s):
+        with pytest.raises(gpu.SolverError, match="was rejected 3 times") as err:
+            s.run(q, 0.5)
+        return str(err.value)
+
+    dev, host = _both_loops(monkeypatch, make, run)
+    assert dev == host
