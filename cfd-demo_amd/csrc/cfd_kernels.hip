@@ -365,20 +365,38 @@ __device__ __forceinline__ void grp_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Group reductions: a group is one 16-lane DPP row, so rotations of the row
+// by 8, 4, 2 and 1 lanes leave every lane with the whole group's result, as
+// VALU moves (a lane shuffle goes through the LDS crossbar, a round trip per
+// step on the repair's dependent chain). Min, max and or do not depend on
+// the order of combination.
+template <int R>
+__device__ __forceinline__ int row_ror(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x120 + R, 0xF, 0xF, false);
+}
+template <int R>
+__device__ __forceinline__ double row_ror(double v) {
+  return __hiloint2double(row_ror<R>(__double2hiint(v)), row_ror<R>(__double2loint(v)));
+}
+static_assert(RG == 16, "group reductions rotate one 16-lane DPP row");
 __device__ __forceinline__ double grp_min(double v) {
-#pragma unroll
-  for (int off = RG / 2; off >= 1; off >>= 1) v = nan_min(v, __shfl_xor(v, off, RG));
-  return v;
+  v = nan_min(v, row_ror<8>(v));
+  v = nan_min(v, row_ror<4>(v));
+  v = nan_min(v, row_ror<2>(v));
+  return nan_min(v, row_ror<1>(v));
 }
 __device__ __forceinline__ double grp_max(double v) {
-#pragma unroll
-  for (int off = RG / 2; off >= 1; off >>= 1) v = nan_max(v, __shfl_xor(v, off, RG));
-  return v;
+  v = nan_max(v, row_ror<8>(v));
+  v = nan_max(v, row_ror<4>(v));
+  v = nan_max(v, row_ror<2>(v));
+  return nan_max(v, row_ror<1>(v));
 }
 __device__ __forceinline__ bool grp_any(bool b) {
   int v = b ? 1 : 0;
-#pragma unroll
-  for (int off = RG / 2; off >= 1; off >>= 1) v |= __shfl_xor(v, off, RG);
+  v |= row_ror<8>(v);
+  v |= row_ror<4>(v);
+  v |= row_ror<2>(v);
+  v |= row_ror<1>(v);
   return v != 0;
 }
 
@@ -533,8 +551,17 @@ __global__ void __launch_bounds__(FBLOCK) __attribute__((amdgpu_waves_per_eu(4))
     return;
   }
   const int g = threadIdx.x / RG, lane = threadIdx.x % RG;
-  for (int idx = blockIdx.x * (FBLOCK / RG) + g; idx < n; idx += gridDim.x * (FBLOCK / RG))
-    repair_group(a, a.work[idx], idx, lane, qs[g], ds[g], row);
+  // A group repairs its entries one after another; the next entry is read
+  // while the current one is repaired, so the list read leaves the chain of
+  // dependent loads each repair starts with.
+  const int stride = gridDim.x * (FBLOCK / RG);
+  int idx = blockIdx.x * (FBLOCK / RG) + g;
+  int entry = idx < n ? a.work[idx] : 0;
+  for (; idx < n; idx += stride) {
+    const int cur = entry;
+    if (idx + stride < n) entry = a.work[idx + stride];
+    repair_group(a, cur, idx, lane, qs[g], ds[g], row);
+  }
   block_reduce_row<FBLOCK>(row, a.part + (int64_t)blockIdx.x * PART_STRIDE);
 }
 
