@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(SB) __attribute__((amdgpu_waves_per_eu(2))) k_
   }
 
   if constexpr (MODE != MODE_RHS) {
-    if (live) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ, K, e);
+    if (live && a.summ) write_summary<MODE == MODE_STAGE3>(acc, gamma, a.summ, K, e);
   }
   if constexpr (MODE != MODE_SUMMARY) block_reduce_bflux<SB>(bfl, a.part + (int64_t)blk * PART_STRIDE);
 }
